@@ -1,0 +1,183 @@
+#!/usr/bin/env python3
+"""Throughput of the fv3d hydro step on one MI355X.
+
+Workload (default): 3D Sedov-style blast, 256^3 cells on the unit cube,
+gamma 5/3, background rho=1 p=0.1, p=10 inside r<0.1, PLM-minmod + HLLC,
+SSP-RK2, CFL 0.8, reflecting walls, FP64.  One step = one full RK2 step
+(two fused stage launches, the second also reducing the next dt) over the
+whole grid, with the state already resident in HBM.
+
+Prints one JSON line:
+  value      Gcell-updates/s (interior cells x steps / s)
+  roofline   the stage kernel: algorithmic bytes per launch (200 B per
+             cell-update over 2 launches) / mean launch time from HIP events
+             recorded on the launch stream, against 8 TB/s
+  traffic    measured HBM bytes per stage launch from the rocprofv3 PMC
+             summary in profiles/ (FETCH_SIZE + WRITE_SIZE), if present
+  cpu_baseline  the CPU restatement of the reference (oracle/, FP64, OpenMP)
+             on the host cores, timed on a bounded window of the same grid
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Gcell-updates/s & %HBM roofline at 256^3 FP64/FP32; adjoint/forward time"
+HBM_PEAK_GBS = 8000.0
+ALG_BYTES_PER_CELL_UPDATE = {"f64": 200.0, "f32": 100.0}
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "stage_kernel_blast256_fp64_traffic.md")
+
+
+def blast_params(n):
+    import fv3d_amd as F
+    from fv3d_amd import Params
+    return Params.defaults().replace(
+        Nx=n, Ny=n, Nz=n, gamma0=5.0 / 3.0, reconstruction=F.PLM, riemann_solver=F.HLLC,
+        time_stepping=F.TS_RK2, CFL=0.8, boundary_x=F.BC_REFLECTING, boundary_y=F.BC_REFLECTING,
+        boundary_z=F.BC_REFLECTING, tend=0.1, save_freq=0.1)
+
+
+def blast_ic(p):
+    from fv3d_amd import problems
+    return problems.blast(p, radius=0.1, rho_in=1.0, p_in=10.0, rho_out=1.0, p_out=0.1)
+
+
+def read_traffic():
+    """HBM bytes per stage launch from the committed rocprofv3 summary."""
+    try:
+        with open(TRAFFIC_SUMMARY) as f:
+            txt = f.read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", txt)
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(n, budget_s):
+    """The oracle (CPU restatement, reference pass structure) on the same
+    grid: one warm-up step, then whole steps until budget_s has elapsed."""
+    from oracle.oracle import Oracle, lib as olib
+    import ctypes
+    p = blast_params(n)
+    o = Oracle(p)
+    Q0 = blast_ic(p)
+    U = o.prim_to_cons(Q0)
+    Q = o.zeros()
+    Unew = o.zeros()
+    dts = np.zeros(64)
+    t = ctypes.c_double()
+    from oracle.oracle import _p
+    L = olib()
+    L.orc_run(ctypes.byref(o.p), _p(U), _p(Q), _p(Unew), _p(o.work), 1, 0, _p(dts), ctypes.byref(t))
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        L.orc_run(ctypes.byref(o.p), _p(U), _p(Q), _p(Unew), _p(o.work), 1, 0, _p(dts), ctypes.byref(t))
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or steps >= 20:
+            break
+    cells = p.Nx * p.Ny * p.Nz
+    return {
+        "value": cells * steps / el / 1e9,
+        "unit": "Gcell-updates/s",
+        "cores": int(L.orc_num_threads()),
+        "kind": "port",
+        "sample": f"blast {n}^3 FP64 RK2, {steps} full steps after 1 warm-up step "
+                  f"({el:.1f} s), reference pass structure, OpenMP",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=256, help="cells per axis")
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU-baseline timing (0: skip)")
+    ap.add_argument("--cpu-n", type=int, default=256, help="cells per axis of the CPU-baseline sample")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is a design note only)", file=sys.stderr)
+        sys.exit(2)
+
+    import fv3d_amd as F
+    p = blast_params(args.n)
+    cells = p.Nx * p.Ny * p.Nz
+    sim = F.Simulation(p, F.FP64)
+    sim.set_prim(blast_ic(p), with_ghosts=True)
+    if args.warmup > 0:
+        sim.step(args.warmup)
+    sim.synchronize()
+    dt0, _ = sim.compute_dt()
+    if not np.isfinite(dt0) or dt0 <= 0:
+        raise RuntimeError("non-finite dt after warm-up")
+
+    # timed window: HIP events on the launch stream around every stage launch
+    t0 = time.perf_counter()
+    total_ms, stage_ms = sim.time_steps(args.steps, per_stage=True)
+    sim.synchronize()
+    wall = time.perf_counter() - t0
+    dt1, _ = sim.compute_dt()
+    if not np.isfinite(dt1) or dt1 <= 0:
+        raise RuntimeError("non-finite dt after the timed window")
+    sim.close()
+
+    ms_per_step = total_ms / args.steps
+    value = cells * args.steps / (total_ms * 1e-3) / 1e9
+    mean_launch_ms = float(np.mean(stage_ms))
+    bytes_per_launch = ALG_BYTES_PER_CELL_UPDATE["f64"] * cells / 2.0
+    achieved = bytes_per_launch / (mean_launch_ms * 1e-3) / 1e9
+    traffic = read_traffic()
+    out = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"3D Sedov-style blast {args.n}^3, gamma 5/3, PLM-minmod + HLLC, SSP-RK2, "
+                        "CFL 0.8, reflecting walls, FP64",
+            "grid": [p.Nx, p.Ny, p.Nz],
+            "step": "one RK2 step = 2 fused stage launches, dt reduced on device",
+            "wall_s_timed_window": round(wall, 4),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "kernel": "fv3d_hip::stage_kernel<double,8,PLM,HLLC,false,false>",
+            "alg_bytes_per_launch": bytes_per_launch,
+            "mean_launch_ms": round(mean_launch_ms, 5),
+        },
+    }
+    if args.cpu_budget > 0:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_budget)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,501 @@
+// fv3d_api.hip -- the C ABI of libfv3d_hip (declared in include/fv3d_hip.h).
+//
+// A context owns the device state of one box: two conserved-state buffers
+// (A = U^n, B = U* of RK2 / the previous state of Euler), a three-slot ring
+// of inv_dt_hyp accumulators and a health flag.  A step is:
+//   RK2:   stage(A -> B)               reads slot s,   clears slot s+2
+//          stage(B, rk=A -> A)         reads slot s,   reduces slot s+1
+//   Euler: stage(A -> B), swap(A, B)   reads slot s,   reduces s+1, clears s+2
+// so dt never round-trips through the host and consecutive steps need no
+// synchronisation.  The slot cleared by step s was last read by step s-1 and
+// is next written by step s+1, so clearing it inside step s races with
+// nothing.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include <algorithm>
+
+#include "fv3d_hip.h"
+#include "fv3d_kernels.h"
+
+using namespace fv3d_hip;
+
+struct fv3d_ctx {
+  fv3d_params p;
+  int precision = FV3D_FP64;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  void* A = nullptr;
+  void* B = nullptr;
+  size_t state_bytes = 0;
+  unsigned long long* ring = nullptr;  // 3 slots
+  int* health = nullptr;
+  double* qbuf = nullptr;  // ghosted SoA primitive staging, allocated on first use
+  long step = 0;
+  bool have_state = false;
+  bool have_stage = false;  // B holds the reference's post-update Q source
+  DevParams dp;
+  std::string err;
+};
+
+static thread_local std::string g_err;
+
+static int fail(fv3d_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(ctx, expr)                                                              \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail((ctx), FV3D_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+static bool is_pow2(double d) {
+  if (!(d > 0.0) || !std::isfinite(d)) return false;
+  int e;
+  return std::frexp(d, &e) == 0.5;
+}
+
+// inv_dt_par for a constant coefficient (ComputeDt.h:37-47)
+static double inv_dt_par(bool active, double coef, const fv3d_params& p) {
+  if (!active) return p.epsilon;
+  return std::fmax(std::fmax(2.0 * coef / (p.dx * p.dx), 2.0 * coef / (p.dy * p.dy)),
+                   2.0 * coef / (p.dz * p.dz));
+}
+
+static int tiles_z_for(const fv3d_params& p, int tiles_xy) {
+  int nzc = 0;
+  if (const char* e = std::getenv("FV3D_NZC")) nzc = std::atoi(e);
+  if (nzc <= 0) {
+    const int target = 512;  // >= 2 workgroups per CU on 256 CUs
+    int tz = (target + tiles_xy - 1) / tiles_xy;
+    tz = std::max(1, std::min(tz, p.Nz / 8 > 0 ? p.Nz / 8 : 1));
+    nzc = (p.Nz + tz - 1) / tz;
+  }
+  return std::max(1, std::min(nzc, p.Nz));
+}
+
+static int check_params(const fv3d_params& p, std::string& why) {
+  if (p.Nx < 2 || p.Ny < 2 || p.Nz < 2) { why = "Nx, Ny, Nz must be >= 2"; return 1; }
+  if (p.Ng < (p.reconstruction == FV3D_PLM ? 2 : 1)) { why = "too few ghost layers"; return 1; }
+  if (p.boundary_z == FV3D_BC_C91) { why = "boundaries_z=C91 is not supported by the fused stage"; return 1; }
+  if (p.thermal_conductivity_active && p.thermal_conductivity_mode != FV3D_TCM_CONSTANT) {
+    why = "conductivity_mode=B02 is not supported";
+    return 1;
+  }
+  if (p.riemann_solver != FV3D_HLL && p.riemann_solver != FV3D_HLLC) { why = "riemann_solver"; return 1; }
+  if (p.reconstruction < FV3D_PCM || p.reconstruction > FV3D_PLM) { why = "reconstruction"; return 1; }
+  if (!(p.dx > 0 && p.dy > 0 && p.dz > 0)) { why = "non-positive cell size"; return 1; }
+  return 0;
+}
+
+static DevParams make_devparams(const fv3d_params& p) {
+  DevParams d;
+  std::memset(&d, 0, sizeof d);
+  d.Nx = p.Nx; d.Ny = p.Ny; d.Nz = p.Nz;
+  d.sy = p.Nx;
+  d.sz = (long long)p.Nx * p.Ny;
+  d.sf = (long long)p.Nx * p.Ny * p.Nz;
+  d.bcx = p.boundary_x; d.bcy = p.boundary_y; d.bcz = p.boundary_z;
+  d.pcm_wb = p.reconstruction == FV3D_PCM_WB;
+  d.wb_flux_zbc = p.well_balanced_flux_at_z_bc != 0;
+  d.c91_flux_zbc = p.boundary_z == FV3D_BC_C91;
+  d.gravity = p.gravity != 0;
+  d.gamma = p.gamma0;
+  d.gm1 = p.gamma0 - 1.0;
+  d.entho = 1.0 / (p.gamma0 - 1.0);
+  d.dx = p.dx; d.dy = p.dy; d.dz = p.dz;
+  d.idx = 1.0 / p.dx; d.idy = 1.0 / p.dy; d.idz = 1.0 / p.dz;
+  d.pow2x = is_pow2(p.dx); d.pow2y = is_pow2(p.dy); d.pow2z = is_pow2(p.dz);
+  d.g = p.g;
+  d.kface = 0.5 * (p.kappa + p.kappa);
+  d.bctc_zmin = p.bctc_zmin; d.bctc_zmax = p.bctc_zmax;
+  d.bctc_zmin_value = p.bctc_zmin_value; d.bctc_zmax_value = p.bctc_zmax_value;
+  d.mu = p.mu;
+  d.CFL = p.CFL;
+  d.inv_tc = inv_dt_par(p.thermal_conductivity_active != 0, p.kappa, p);
+  d.inv_visc = inv_dt_par(p.viscosity_active != 0, p.mu, p);
+  d.tiles_x = (p.Nx + STAGE_WX - 1) / STAGE_WX;
+  d.tiles_y = (p.Ny + STAGE_TY - 1) / STAGE_TY;
+  d.nzc = tiles_z_for(p, d.tiles_x * d.tiles_y);
+  d.tiles_z = (p.Nz + d.nzc - 1) / d.nzc;
+  return d;
+}
+
+extern "C" {
+
+int fv3d_abi_version(void) { return 1; }
+size_t fv3d_params_size(void) { return sizeof(fv3d_params); }
+
+const char* fv3d_last_error(const fv3d_ctx* ctx) {
+  return ctx ? ctx->err.c_str() : g_err.c_str();
+}
+
+int fv3d_create(const fv3d_params* p, int precision, int device, fv3d_ctx** out) {
+  if (!p || !out) return fail(nullptr, FV3D_EINVAL, "null argument");
+  *out = nullptr;
+  std::string why;
+  if (check_params(*p, why)) return fail(nullptr, FV3D_EINVAL, why);
+  if (precision != FV3D_FP64) return fail(nullptr, FV3D_EINVAL, "precision not supported");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(nullptr, FV3D_EHIP, "no HIP device visible");
+  if (device < 0 || device >= ndev) return fail(nullptr, FV3D_EINVAL, "device index out of range");
+  fv3d_ctx* c = new fv3d_ctx();
+  c->p = *p;
+  c->precision = precision;
+  c->device = device;
+  c->dp = make_devparams(*p);
+  const size_t elem = sizeof(double);
+  c->state_bytes = (size_t)5 * p->Nx * p->Ny * p->Nz * elem;
+  auto cleanup = [&](int code, const std::string& m) {
+    fv3d_destroy(c);
+    return fail(nullptr, code, m);
+  };
+  if (hipSetDevice(device) != hipSuccess) return cleanup(FV3D_EHIP, "hipSetDevice failed");
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    return cleanup(FV3D_EHIP, "hipStreamCreate failed");
+  if (hipMalloc(&c->A, c->state_bytes) != hipSuccess ||
+      hipMalloc(&c->B, c->state_bytes) != hipSuccess)
+    return cleanup(FV3D_ENOMEM, "state allocation failed");
+  if (hipMalloc(&c->ring, 4 * sizeof(unsigned long long)) != hipSuccess ||
+      hipMalloc(&c->health, sizeof(int)) != hipSuccess)
+    return cleanup(FV3D_ENOMEM, "allocation failed");
+  if (hipMemsetAsync(c->ring, 0, 4 * sizeof(unsigned long long), c->stream) != hipSuccess ||
+      hipMemsetAsync(c->health, 0, sizeof(int), c->stream) != hipSuccess)
+    return cleanup(FV3D_EHIP, "memset failed");
+  *out = c;
+  return FV3D_OK;
+}
+
+int fv3d_destroy(fv3d_ctx* c) {
+  if (!c) return FV3D_OK;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->A) hipFree(c->A);
+  if (c->B) hipFree(c->B);
+  if (c->ring) hipFree(c->ring);
+  if (c->health) hipFree(c->health);
+  if (c->qbuf) hipFree(c->qbuf);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return FV3D_OK;
+}
+
+}  // extern "C"
+
+// ---- host layout helpers --------------------------------------------------
+// host AoS (k, j, i, var) with or without ghosts  <->  SoA domain [5][Nz][Ny][Nx]
+static void aos_to_soa_domain(const fv3d_params& p, const double* src, int with_ghosts,
+                              std::vector<double>& dst) {
+  const long long n = (long long)p.Nx * p.Ny * p.Nz;
+  dst.resize(5 * n);
+  const int g = with_ghosts ? p.Ng : 0;
+  const long long tx = p.Nx + 2 * g, ty = p.Ny + 2 * g;
+  for (int k = 0; k < p.Nz; ++k)
+    for (int j = 0; j < p.Ny; ++j)
+      for (int i = 0; i < p.Nx; ++i) {
+        const double* s = src + (((long long)(k + g) * ty + (j + g)) * tx + (i + g)) * 5;
+        const long long d = i + (long long)p.Nx * (j + (long long)p.Ny * k);
+        for (int f = 0; f < 5; ++f) dst[f * n + d] = s[f];
+      }
+}
+
+static void soa_domain_to_aos(const fv3d_params& p, const double* src, int with_ghosts,
+                              double* dst) {
+  const long long n = (long long)p.Nx * p.Ny * p.Nz;
+  const int g = with_ghosts ? p.Ng : 0;
+  const long long tx = p.Nx + 2 * g, ty = p.Ny + 2 * g;
+  for (int k = 0; k < p.Nz; ++k)
+    for (int j = 0; j < p.Ny; ++j)
+      for (int i = 0; i < p.Nx; ++i) {
+        double* d = dst + (((long long)(k + g) * ty + (j + g)) * tx + (i + g)) * 5;
+        const long long s = i + (long long)p.Nx * (j + (long long)p.Ny * k);
+        for (int f = 0; f < 5; ++f) d[f] = src[f * n + s];
+      }
+}
+
+// ghosted SoA [5][Ntz][Nty][Ntx]  <->  ghosted AoS (Ntz, Nty, Ntx, 5)
+static void soa_ghosted_to_aos(const fv3d_params& p, const double* src, double* dst) {
+  const long long nt = (long long)p.Ntx * p.Nty * p.Ntz;
+  for (long long c = 0; c < nt; ++c)
+    for (int f = 0; f < 5; ++f) dst[c * 5 + f] = src[f * nt + c];
+}
+static void aos_to_soa_ghosted(const fv3d_params& p, const double* src, double* dst) {
+  const long long nt = (long long)p.Ntx * p.Nty * p.Ntz;
+  for (long long c = 0; c < nt; ++c)
+    for (int f = 0; f < 5; ++f) dst[f * nt + c] = src[c * 5 + f];
+}
+
+static int ensure_qbuf(fv3d_ctx* c) {
+  if (c->qbuf) return FV3D_OK;
+  const size_t nt = (size_t)c->p.Ntx * c->p.Nty * c->p.Ntz;
+  if (hipMalloc(&c->qbuf, 5 * nt * sizeof(double)) != hipSuccess)
+    return fail(c, FV3D_ENOMEM, "staging allocation failed");
+  return FV3D_OK;
+}
+
+// prim of a device state buffer into host memory
+static int state_prim_to_host(fv3d_ctx* c, const void* U, double* Q, int with_ghosts) {
+  int rc = ensure_qbuf(c);
+  if (rc) return rc;
+  const fv3d_params& p = c->p;
+  const size_t nt = (size_t)p.Ntx * p.Nty * p.Ntz;
+  HIPCHK(c, launch_cons_to_prim_ghosted<double>((const double*)U, c->qbuf, c->dp, p.Ng, c->stream));
+  if (with_ghosts) HIPCHK(c, launch_fill_boundaries(c->qbuf, p, c->stream));
+  std::vector<double> h(5 * nt);
+  HIPCHK(c, hipMemcpyAsync(h.data(), c->qbuf, 5 * nt * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (with_ghosts) {
+    soa_ghosted_to_aos(p, h.data(), Q);
+  } else {
+    const long long tx = p.Ntx, ty = p.Nty;
+    for (int k = 0; k < p.Nz; ++k)
+      for (int j = 0; j < p.Ny; ++j)
+        for (int i = 0; i < p.Nx; ++i) {
+          const long long s = (i + p.Ng) + tx * ((j + p.Ng) + ty * (k + p.Ng));
+          double* d = Q + (((long long)k * p.Ny + j) * p.Nx + i) * 5;
+          for (int f = 0; f < 5; ++f) d[f] = h[f * nt + s];
+        }
+  }
+  return FV3D_OK;
+}
+
+static int reset_dt(fv3d_ctx* c) {
+  HIPCHK(c, launch_fill_zero_u64(c->ring, 4, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->health, 0, sizeof(int), c->stream));
+  HIPCHK(c, launch_dt_reduce<double>((const double*)c->A, c->ring, c->health, c->dp, c->stream));
+  c->step = 0;
+  c->have_state = true;
+  c->have_stage = false;
+  return FV3D_OK;
+}
+
+extern "C" {
+
+int fv3d_set_prim(fv3d_ctx* c, const double* Q, int with_ghosts) {
+  if (!c || !Q) return fail(c, FV3D_EINVAL, "null argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  std::vector<double> h;
+  aos_to_soa_domain(c->p, Q, with_ghosts, h);
+  double* d = nullptr;
+  HIPCHK(c, hipMalloc(&d, h.size() * sizeof(double)));
+  hipError_t e = hipMemcpyAsync(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
+                                c->stream);
+  if (e == hipSuccess) e = launch_prim_to_cons<double>(d, (double*)c->A, c->dp, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  hipFree(d);
+  if (e != hipSuccess) return fail(c, FV3D_EHIP, std::string("set_prim: ") + hipGetErrorString(e));
+  return reset_dt(c);
+}
+
+int fv3d_set_cons(fv3d_ctx* c, const double* U, int with_ghosts) {
+  if (!c || !U) return fail(c, FV3D_EINVAL, "null argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  std::vector<double> h;
+  aos_to_soa_domain(c->p, U, with_ghosts, h);
+  HIPCHK(c, hipMemcpyAsync(c->A, h.data(), c->state_bytes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return reset_dt(c);
+}
+
+int fv3d_get_cons(fv3d_ctx* c, double* U, int with_ghosts) {
+  if (!c || !U) return fail(c, FV3D_EINVAL, "null argument");
+  if (!c->have_state) return fail(c, FV3D_EINVAL, "no state set");
+  HIPCHK(c, hipSetDevice(c->device));
+  std::vector<double> h(c->state_bytes / sizeof(double));
+  HIPCHK(c, hipMemcpyAsync(h.data(), c->A, c->state_bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  soa_domain_to_aos(c->p, h.data(), with_ghosts, U);
+  return FV3D_OK;
+}
+
+int fv3d_get_prim(fv3d_ctx* c, double* Q, int with_ghosts) {
+  if (!c || !Q) return fail(c, FV3D_EINVAL, "null argument");
+  if (!c->have_state) return fail(c, FV3D_EINVAL, "no state set");
+  HIPCHK(c, hipSetDevice(c->device));
+  return state_prim_to_host(c, c->A, Q, with_ghosts);
+}
+
+int fv3d_get_stage_prim(fv3d_ctx* c, double* Q, int with_ghosts) {
+  if (!c || !Q) return fail(c, FV3D_EINVAL, "null argument");
+  if (!c->have_stage) return fail(c, FV3D_EINVAL, "no step taken yet");
+  HIPCHK(c, hipSetDevice(c->device));
+  return state_prim_to_host(c, c->B, Q, with_ghosts);
+}
+
+int fv3d_compute_dt(fv3d_ctx* c, double inv[3], double* dt) {
+  if (!c) return fail(c, FV3D_EINVAL, "null argument");
+  if (!c->have_state) return fail(c, FV3D_EINVAL, "no state set");
+  HIPCHK(c, hipSetDevice(c->device));
+  unsigned long long bits = 0;
+  int health = 0;
+  HIPCHK(c, hipMemcpyAsync(&bits, c->ring + (c->step % 3), sizeof bits, hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipMemcpyAsync(&health, c->health, sizeof health, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double ih;
+  std::memcpy(&ih, &bits, sizeof ih);
+  double m = ih;
+  if (m < c->dp.inv_tc) m = c->dp.inv_tc;
+  if (m < c->dp.inv_visc) m = c->dp.inv_visc;
+  if (inv) { inv[0] = ih; inv[1] = c->dp.inv_tc; inv[2] = c->dp.inv_visc; }
+  if (dt) *dt = c->p.CFL / m;
+  if (health) return fail(c, FV3D_ENONFINITE, "non-finite inverse time step in the state");
+  return FV3D_OK;
+}
+
+}  // extern "C"
+
+static int launch_step(fv3d_ctx* c, hipEvent_t* ev) {
+  const long s = c->step;
+  unsigned long long* cur = c->ring + (s % 3);
+  unsigned long long* nxt = c->ring + ((s + 1) % 3);
+  unsigned long long* zer = c->ring + ((s + 2) % 3);
+  StageLaunch<double> L;
+  L.recon = c->p.reconstruction == FV3D_PLM ? FV3D_PLM : FV3D_PCM;
+  L.riemann = c->p.riemann_solver;
+  L.tc = c->p.thermal_conductivity_active != 0;
+  L.visc = c->p.viscosity_active != 0;
+  L.health = c->health;
+  L.P = c->dp;
+  L.inv_cur = cur;
+  double* A = (double*)c->A;
+  double* B = (double*)c->B;
+  if (c->p.time_stepping == FV3D_TS_RK2) {
+    L.Uin = A; L.Urk = nullptr; L.Uout = B; L.inv_next = nullptr; L.inv_zero = zer;
+    if (ev) HIPCHK(c, hipEventRecord(ev[0], c->stream));
+    HIPCHK(c, launch_stage<double>(L, c->stream));
+    if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
+    L.Uin = B; L.Urk = A; L.Uout = A; L.inv_next = nxt; L.inv_zero = nullptr;
+    HIPCHK(c, launch_stage<double>(L, c->stream));
+    if (ev) HIPCHK(c, hipEventRecord(ev[2], c->stream));
+  } else {
+    L.Uin = A; L.Urk = nullptr; L.Uout = B; L.inv_next = nxt; L.inv_zero = zer;
+    if (ev) HIPCHK(c, hipEventRecord(ev[0], c->stream));
+    HIPCHK(c, launch_stage<double>(L, c->stream));
+    if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
+    std::swap(c->A, c->B);
+  }
+  c->step = s + 1;
+  c->have_stage = true;
+  return FV3D_OK;
+}
+
+extern "C" {
+
+int fv3d_step(fv3d_ctx* c) {
+  if (!c) return fail(c, FV3D_EINVAL, "null argument");
+  if (!c->have_state) return fail(c, FV3D_EINVAL, "no state set");
+  HIPCHK(c, hipSetDevice(c->device));
+  return launch_step(c, nullptr);
+}
+
+int fv3d_steps(fv3d_ctx* c, int nsteps) {
+  if (!c) return fail(c, FV3D_EINVAL, "null argument");
+  if (!c->have_state) return fail(c, FV3D_EINVAL, "no state set");
+  HIPCHK(c, hipSetDevice(c->device));
+  for (int i = 0; i < nsteps; ++i) {
+    int rc = launch_step(c, nullptr);
+    if (rc) return rc;
+  }
+  return FV3D_OK;
+}
+
+long fv3d_step_count(const fv3d_ctx* c) { return c ? c->step : -1; }
+
+int fv3d_synchronize(fv3d_ctx* c) {
+  if (!c) return fail(c, FV3D_EINVAL, "null argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return FV3D_OK;
+}
+
+int fv3d_time_steps(fv3d_ctx* c, int nsteps, float* total_ms, float* stage_ms) {
+  if (!c || nsteps <= 0) return fail(c, FV3D_EINVAL, "bad argument");
+  if (!c->have_state) return fail(c, FV3D_EINVAL, "no state set");
+  HIPCHK(c, hipSetDevice(c->device));
+  const int per = c->p.time_stepping == FV3D_TS_RK2 ? 3 : 2;
+  std::vector<hipEvent_t> ev(stage_ms ? (size_t)per * nsteps : 0);
+  hipEvent_t t0, t1;
+  HIPCHK(c, hipEventCreate(&t0));
+  HIPCHK(c, hipEventCreate(&t1));
+  for (auto& e : ev) HIPCHK(c, hipEventCreate(&e));
+  HIPCHK(c, hipEventRecord(t0, c->stream));
+  int rc = FV3D_OK;
+  for (int i = 0; i < nsteps && rc == FV3D_OK; ++i)
+    rc = launch_step(c, stage_ms ? &ev[(size_t)per * i] : nullptr);
+  HIPCHK(c, hipEventRecord(t1, c->stream));
+  HIPCHK(c, hipEventSynchronize(t1));
+  if (rc == FV3D_OK) {
+    if (total_ms) HIPCHK(c, hipEventElapsedTime(total_ms, t0, t1));
+    if (stage_ms) {
+      int o = 0;
+      for (int i = 0; i < nsteps; ++i)
+        for (int s = 0; s + 1 < per; ++s)
+          HIPCHK(c, hipEventElapsedTime(&stage_ms[o++], ev[(size_t)per * i + s],
+                                        ev[(size_t)per * i + s + 1]));
+    }
+  }
+  for (auto& e : ev) hipEventDestroy(e);
+  hipEventDestroy(t0);
+  hipEventDestroy(t1);
+  return rc;
+}
+
+int fv3d_fill_boundaries(const fv3d_params* p, double* Q) {
+  if (!p || !Q) return fail(nullptr, FV3D_EINVAL, "null argument");
+  const size_t nt = (size_t)p->Ntx * p->Nty * p->Ntz;
+  std::vector<double> h(5 * nt);
+  aos_to_soa_ghosted(*p, Q, h.data());
+  double* d = nullptr;
+  hipStream_t s;
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess)
+    return fail(nullptr, FV3D_EHIP, "hipStreamCreate failed");
+  hipError_t e = hipMalloc(&d, 5 * nt * sizeof(double));
+  if (e == hipSuccess) e = hipMemcpyAsync(d, h.data(), 5 * nt * sizeof(double), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = launch_fill_boundaries(d, *p, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(h.data(), d, 5 * nt * sizeof(double), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (d) hipFree(d);
+  hipStreamDestroy(s);
+  if (e != hipSuccess) return fail(nullptr, FV3D_EHIP, std::string("fill_boundaries: ") + hipGetErrorString(e));
+  soa_ghosted_to_aos(*p, h.data(), Q);
+  return FV3D_OK;
+}
+
+int fv3d_riemann(int solver, double gamma0, long n, const double* qL, const double* qR,
+                 double* flux, double* pout) {
+  if (n < 0 || (n > 0 && (!qL || !qR || !flux || !pout)))
+    return fail(nullptr, FV3D_EINVAL, "bad argument");
+  if (solver != FV3D_HLL && solver != FV3D_HLLC) return fail(nullptr, FV3D_EINVAL, "solver");
+  if (n == 0) return FV3D_OK;
+  double *dl = nullptr, *dr = nullptr, *df = nullptr, *dp = nullptr;
+  hipStream_t s;
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess)
+    return fail(nullptr, FV3D_EHIP, "hipStreamCreate failed");
+  const size_t b5 = (size_t)n * 5 * sizeof(double), b1 = (size_t)n * sizeof(double);
+  hipError_t e = hipMalloc(&dl, b5);
+  if (e == hipSuccess) e = hipMalloc(&dr, b5);
+  if (e == hipSuccess) e = hipMalloc(&df, b5);
+  if (e == hipSuccess) e = hipMalloc(&dp, b1);
+  if (e == hipSuccess) e = hipMemcpyAsync(dl, qL, b5, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(dr, qR, b5, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = launch_riemann_batch(solver, gamma0, n, dl, dr, df, dp, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(flux, df, b5, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(pout, dp, b1, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  for (double* x : {dl, dr, df, dp})
+    if (x) hipFree(x);
+  hipStreamDestroy(s);
+  if (e != hipSuccess) return fail(nullptr, FV3D_EHIP, std::string("riemann: ") + hipGetErrorString(e));
+  return FV3D_OK;
+}
+
+}  // extern "C"

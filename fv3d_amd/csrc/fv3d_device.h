@@ -1,0 +1,233 @@
+// fv3d_device.h -- device-side state algebra, Riemann solvers and ghost maps
+// for the CDNA4 kernels of libfv3d_hip.
+//
+// Every expression keeps the evaluation order of the reference (C++ left to
+// right, no contraction: the library is built with -ffp-contract=off), so the
+// FP64 results are the reference's bit for bit.  The references next to each
+// function give the file:line whose arithmetic it restates.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "fv3d_params.h"
+
+namespace fv3d_hip {
+
+// Kernel-constant parameters, passed by value (kernarg segment).
+struct DevParams {
+  int Nx, Ny, Nz;
+  long long sy, sz, sf;         // element strides of the SoA state: x=1, y, z, field
+  int bcx, bcy, bcz;            // BoundaryType per axis
+  int pcm_wb;                   // PCM_WB pressure shift on z faces (Update.h:22-28)
+  int wb_flux_zbc;              // well_balanced_flux_at_z_bc (Update.h:132-137)
+  int c91_flux_zbc;             // boundary_z == BC_C91 flux override (Update.h:138-143)
+  int gravity;
+  int pow2x, pow2y, pow2z;      // d is a power of two: x/d == x*(1/d) exactly
+  double gamma, gm1, entho;     // gamma0, gamma0-1.0, 1.0/(gamma0-1.0)
+  double dx, dy, dz, idx, idy, idz;
+  double g;
+  double kface;                 // 0.5*(kappa+kappa): constant-mode face conductivity
+  int bctc_zmin, bctc_zmax;
+  double bctc_zmin_value, bctc_zmax_value;
+  double mu;
+  double CFL, inv_tc, inv_visc; // dt inputs; inv_tc/inv_visc are constants of the run
+  int tiles_x, tiles_y, tiles_z, nzc;
+};
+
+template <typename real>
+struct Prim {
+  real r, u, v, w, p;
+};
+
+// ---------------------------------------------------------------------------
+// state algebra
+// ---------------------------------------------------------------------------
+
+// States.h:34-45 (consToPrim); gm1 = gamma0-1.0
+template <typename real>
+__device__ __forceinline__ Prim<real> cons_to_prim(real U0, real U1, real U2, real U3, real U4,
+                                                   real gm1) {
+  Prim<real> q;
+  q.r = U0;
+  q.u = U1 / U0;
+  q.v = U2 / U0;
+  q.w = U3 / U0;
+  const real Ek = (real)0.5 * q.r * (q.u * q.u + q.v * q.v + q.w * q.w);
+  q.p = (U4 - Ek) * gm1;
+  return q;
+}
+
+// States.h:20-31 (primToCons)
+template <typename real>
+__device__ __forceinline__ void prim_to_cons(const Prim<real>& q, real gm1, real U[5]) {
+  U[0] = q.r;
+  U[1] = q.r * q.u;
+  U[2] = q.r * q.v;
+  U[3] = q.r * q.w;
+  const real Ek = (real)0.5 * (U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) / q.r;
+  U[4] = (Ek + q.p / gm1);
+}
+
+// States.h:47-50
+template <typename real>
+__device__ __forceinline__ real sound_speed(real r, real p, real gamma) {
+  return sqrt(p * gamma / r);
+}
+
+// Update.h:73-80: minmod(a, b) = a*b<0 ? 0 : (|a|<|b| ? a : b).  The product
+// (not a sign test) decides, as in the reference.
+template <typename real>
+__device__ __forceinline__ real minmod(real a, real b) {
+  return (a * b < (real)0) ? (real)0 : ((fabs(a) < fabs(b)) ? a : b);
+}
+
+// ---------------------------------------------------------------------------
+// Riemann solvers.  Inputs are face states already rotated into the face
+// frame (swap_component, States.h:103-111): slot u is the normal velocity.
+// ---------------------------------------------------------------------------
+
+// RiemannSolvers.h:38-112.  Only the selected star side is evaluated; each
+// value that is used is produced by the reference's own expression.
+template <typename real>
+__device__ __forceinline__ void hllc(const Prim<real>& L, const Prim<real>& R, real gamma,
+                                     real entho, real F[5], real& pout) {
+  const real ekL = (real)0.5 * L.r * (L.u * L.u + L.v * L.v);  // drops the 2nd transverse velocity
+  const real EL = ekL + L.p * entho;
+  const real ekR = (real)0.5 * R.r * (R.u * R.u + R.v * R.v);
+  const real ER = ekR + R.p * entho;
+  const real cL = sound_speed(L.r, L.p, gamma);
+  const real cR = sound_speed(R.r, R.p, gamma);
+  const real cmax = fmax(cL, cR);
+  const real SL = fmin(L.u, R.u) - cmax;
+  const real SR = fmax(L.u, R.u) + cmax;
+  const real rcL = L.r * (L.u - SL);
+  const real rcR = R.r * (SR - R.u);
+  const real uS = (rcR * R.u + rcL * L.u + (L.p - R.p)) / (rcR + rcL);
+  const real pS = (rcR * L.p + rcL * R.p + rcL * rcR * (L.u - R.u)) / (rcR + rcL);
+  // star side K: left when uS > 0, else right
+  const bool kl = uS > (real)0;
+  const real rK = kl ? L.r : R.r, uK = kl ? L.u : R.u, pK = kl ? L.p : R.p;
+  const real EK = kl ? EL : ER, SK = kl ? SL : SR;
+  const real vK = kl ? L.v : R.v, wK = kl ? L.w : R.w;
+  const real rS = rK * (SK - uK) / (SK - uS);
+  const real ES = ((SK - uK) * EK - pK * uK + pS * uS) / (SK - uS);
+  const bool star = !(SL > (real)0) && (kl || SR > (real)0);
+  const bool left = SL > (real)0;
+  real sr, su, sv, sw, sp, E;
+  if (star) {
+    sr = rS; su = uS; sv = vK; sw = wK; sp = pS; E = ES;
+  } else if (left) {
+    sr = L.r; su = L.u; sv = L.v; sw = L.w; sp = L.p; E = EL;
+  } else {
+    sr = R.r; su = R.u; sv = R.v; sw = R.w; sp = R.p; E = ER;
+  }
+  pout = sp;
+  F[0] = sr * su;
+  F[1] = sr * su * su + sp;
+  F[2] = F[0] * sv;
+  F[3] = F[0] * sw;
+  F[4] = (E + sp) * su;
+}
+
+// RiemannSolvers.h:5-36 with computeFlux (States.h:113-127) and primToCons
+// (States.h:20-31).  p/(gamma0-1) appears in both and is computed once.
+template <typename real>
+__device__ __forceinline__ void hll(const Prim<real>& L, const Prim<real>& R, real gamma, real gm1,
+                                    real F[5], real& pout) {
+  const real aL = sound_speed(L.r, L.p, gamma);
+  const real aR = sound_speed(R.r, R.p, gamma);
+  const real SL = fmin(L.u - aL, R.u - aR);
+  const real SR = fmax(L.u + aL, R.u + aR);
+  const real peL = L.p / gm1, peR = R.p / gm1;
+  const real EkL = (real)0.5 * L.r * (L.u * L.u + L.v * L.v + L.w * L.w);
+  const real EkR = (real)0.5 * R.r * (R.u * R.u + R.v * R.v + R.w * R.w);
+  const real EfL = (peL + EkL), EfR = (peR + EkR);
+  real FL[5], FR[5];
+  FL[0] = L.r * L.u; FL[1] = L.r * L.u * L.u + L.p; FL[2] = L.r * L.u * L.v;
+  FL[3] = L.r * L.u * L.w; FL[4] = (L.p + EfL) * L.u;
+  FR[0] = R.r * R.u; FR[1] = R.r * R.u * R.u + R.p; FR[2] = R.r * R.u * R.v;
+  FR[3] = R.r * R.u * R.w; FR[4] = (R.p + EfR) * R.u;
+  if (SL >= (real)0) {
+#pragma unroll
+    for (int n = 0; n < 5; ++n) F[n] = FL[n];
+    pout = L.p;
+  } else if (SR <= (real)0) {
+#pragma unroll
+    for (int n = 0; n < 5; ++n) F[n] = FR[n];
+    pout = R.p;
+  } else {
+    real UL[5], UR[5];
+    UL[0] = L.r; UL[1] = L.r * L.u; UL[2] = L.r * L.v; UL[3] = L.r * L.w;
+    UL[4] = ((real)0.5 * (UL[1] * UL[1] + UL[2] * UL[2] + UL[3] * UL[3]) / L.r + peL);
+    UR[0] = R.r; UR[1] = R.r * R.u; UR[2] = R.r * R.v; UR[3] = R.r * R.w;
+    UR[4] = ((real)0.5 * (UR[1] * UR[1] + UR[2] * UR[2] + UR[3] * UR[3]) / R.r + peR);
+    pout = (real)0.5 * (L.p + R.p);
+    const real SLSR = SL * SR, dS = SR - SL;
+#pragma unroll
+    for (int n = 0; n < 5; ++n) F[n] = (FL[n] * SR - FR[n] * SL + (UR[n] - UL[n]) * SLSR) / dS;
+  }
+}
+
+template <typename real, int RS>
+__device__ __forceinline__ void riemann(const Prim<real>& L, const Prim<real>& R, real gamma,
+                                        real gm1, real entho, real F[5], real& pout) {
+  if (RS == FV3D_HLL) hll(L, R, gamma, gm1, F, pout);
+  else hllc(L, R, gamma, entho, F, pout);
+}
+
+// swap_component (States.h:103-111) on a state in the face frame
+template <typename real, int DIR>
+__device__ __forceinline__ Prim<real> rotate(const Prim<real>& q) {
+  if (DIR == FV3D_IX) return q;
+  if (DIR == FV3D_IY) return Prim<real>{q.r, q.v, q.u, q.w, q.p};
+  return Prim<real>{q.r, q.w, q.v, q.u, q.p};
+}
+
+template <typename real, int DIR>
+__device__ __forceinline__ void unrotate_flux(real F[5]) {
+  if (DIR == FV3D_IY) { real t = F[1]; F[1] = F[2]; F[2] = t; }
+  if (DIR == FV3D_IZ) { real t = F[1]; F[1] = F[3]; F[3] = t; }
+}
+
+// ---------------------------------------------------------------------------
+// ghost maps.  fillBoundaries (BoundaryConditions.h:146-215) runs X, Y, Z
+// passes; for absorbing/reflecting/periodic each pass changes one index and
+// at most negates one velocity, so the value of any ghost (edges and corners
+// included) is the cell reached by mapping each coordinate on its own, with
+// the normal velocity of every reflected axis negated.  C91 in x or y takes
+// the periodic branch of those passes (BoundaryConditions.h:373,394).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int map_coord(int i, int N, int bc, bool& flip) {
+  flip = false;
+  int s = i;
+  if (i < 0) {
+    if (bc == FV3D_BC_ABSORBING) s = 0;
+    else if (bc == FV3D_BC_REFLECTING) { s = -i - 1; flip = true; }
+    else s = i + N;
+  } else if (i >= N) {
+    if (bc == FV3D_BC_ABSORBING) s = N - 1;
+    else if (bc == FV3D_BC_REFLECTING) { s = 2 * N - i - 1; flip = true; }
+    else s = i - N;
+  }
+  // cells further out than the 2-deep ghost shell only occur in the overhang
+  // of partial tiles and are never used; keep their address in range
+  s = s < 0 ? 0 : (s >= N ? N - 1 : s);
+  return s;
+}
+
+__device__ __forceinline__ double u64_as_double(unsigned long long b) {
+  return __longlong_as_double((long long)b);
+}
+__device__ __forceinline__ unsigned long long double_as_u64(double d) {
+  return (unsigned long long)__double_as_longlong(d);
+}
+
+// dt = CFL / max(inv_dt_hyp, inv_dt_par_tc, inv_dt_par_visc)  (ComputeDt.h:62)
+__device__ __forceinline__ double dt_from_inv(double ih, const DevParams& P) {
+  double m = ih;
+  if (m < P.inv_tc) m = P.inv_tc;
+  if (m < P.inv_visc) m = P.inv_visc;
+  return P.CFL / m;
+}
+
+}  // namespace fv3d_hip

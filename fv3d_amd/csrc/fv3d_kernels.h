@@ -1,0 +1,50 @@
+// fv3d_kernels.h -- host-side launch interface of the libfv3d_hip kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include "fv3d_device.h"
+
+namespace fv3d_hip {
+
+// rows per stage tile (the tile is 64 x STAGE_TY columns)
+constexpr int STAGE_TY = 8;
+constexpr int STAGE_WX = 64;
+
+template <typename real>
+struct StageLaunch {
+  const real* Uin;
+  const real* Urk;  // RK2 combine partner (U^n) or nullptr
+  real* Uout;
+  const unsigned long long* inv_cur;
+  unsigned long long* inv_next;  // nullptr: no dt reduction
+  unsigned long long* inv_zero;  // nullptr: nothing to clear
+  int* health;
+  int recon, riemann;
+  bool tc, visc;
+  DevParams P;
+};
+
+template <typename real>
+hipError_t launch_stage(const StageLaunch<real>& L, hipStream_t s);
+
+// cons -> prim over the domain, written into a ghosted SoA array
+// [5][Ntz][Nty][Ntx] (ghosts untouched)
+template <typename real>
+hipError_t launch_cons_to_prim_ghosted(const real* U, double* Q, const DevParams& P, int Ng,
+                                       hipStream_t s);
+// prim (SoA domain [5][Nz][Ny][Nx], double) -> cons state
+template <typename real>
+hipError_t launch_prim_to_cons(const double* Q, real* U, const DevParams& P, hipStream_t s);
+// ghost fill of a ghosted SoA prim array: three ordered passes X, Y, Z
+hipError_t launch_fill_boundaries(double* Q, const fv3d_params& p, hipStream_t s);
+// max of inv_dt_hyp over the domain of U, into *inv (must be zeroed first)
+template <typename real>
+hipError_t launch_dt_reduce(const real* U, unsigned long long* inv, int* health,
+                            const DevParams& P, hipStream_t s);
+// batched Riemann solves over [n][5] states (face frame)
+hipError_t launch_riemann_batch(int solver, double gamma, long n, const double* qL,
+                                const double* qR, double* flux, double* pout, hipStream_t s);
+// copy of n elements with a cast (state conversions between precisions)
+hipError_t launch_fill_zero_u64(unsigned long long* p, int n, hipStream_t s);
+
+}  // namespace fv3d_hip

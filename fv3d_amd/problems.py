@@ -1,0 +1,181 @@
+"""Host-side problem generators (Init.h) and the benchmarked workload ICs.
+
+These run once per run on the host and return a primitive array in the
+reference's host layout (Ntz, Nty, Ntx, 5); ghost cells are left at zero
+(the device builds ghosts from the boundary maps).  The reference's own
+problems keep its expressions (Init.h:18-209), so Sod and blast are
+bit-identical to the reference's initial conditions.
+
+Random perturbations (H84, C91, the workload ICs) come from a counter-based
+generator keyed by (seed, cell) -- splitmix64 (Steele, Lea, Flood 2014) --
+so they do not depend on the thread mapping, unlike Kokkos'
+Random_XorShift64_Pool in the reference (Init.h:114-116).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+PROBLEMS = {"sod_x": 0, "sod_y": 1, "sod_z": 2, "blast": 3, "rayleigh-taylor": 4,
+            "diffusion": 5, "H84": 6, "C91": 7}
+
+_M64 = (1 << 64) - 1
+
+
+def splitmix64(x: np.ndarray) -> np.ndarray:
+    x = (x + np.uint64(0x9E3779B97F4A7C15)) & np.uint64(_M64)
+    x = ((x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)) & np.uint64(_M64)
+    x = ((x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)) & np.uint64(_M64)
+    return x ^ (x >> np.uint64(31))
+
+
+def cell_uniform(seed: int, cells: np.ndarray, lo: float, hi: float, stream: int = 0) -> np.ndarray:
+    """U[lo, hi) per cell id, keyed by (seed, stream, cell)."""
+    with np.errstate(over="ignore"):
+        key = (np.uint64(seed & 0xFFFFFF) << np.uint64(40)) ^ np.uint64(stream << 36) ^ \
+            cells.astype(np.uint64)
+        r = splitmix64(key)
+    u = (r >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    return lo + (hi - lo) * u
+
+
+def _grid(p):
+    """Cell-centre coordinates of the domain (getPos, SimInfo.h:162-167) and cell ids."""
+    i = np.arange(p.ibeg, p.iend)
+    j = np.arange(p.jbeg, p.jend)
+    k = np.arange(p.kbeg, p.kend)
+    x = p.xmin + ((i - p.ibeg) + 0.5) * p.dx
+    y = p.ymin + ((j - p.jbeg) + 0.5) * p.dy
+    z = p.zmin + ((k - p.kbeg) + 0.5) * p.dz
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    K, J, I = np.meshgrid(k, j, i, indexing="ij")
+    cells = (K.astype(np.int64) * p.Nty + J) * p.Ntx + I
+    return X, Y, Z, cells
+
+
+def _empty(p):
+    return np.zeros((p.Ntz, p.Nty, p.Ntx, 5))
+
+
+def _dom(Q, p):
+    return Q[p.kbeg:p.kend, p.jbeg:p.jend, p.ibeg:p.iend]
+
+
+def sod(p, axis: int = 0):
+    """initSodX/Y/Z (Init.h:18-67)."""
+    Q = _empty(p)
+    c = _grid(p)[axis]
+    left = c <= 0.5
+    d = _dom(Q, p)
+    d[..., 0] = np.where(left, 1.0, 0.125)
+    d[..., 4] = np.where(left, 1.0, 0.1)
+    return Q
+
+
+def blast(p, radius=0.2, rho_in=1.0, p_in=10.0, rho_out=1.2, p_out=0.1):
+    """initBlast (Init.h:72-99); the defaults are the reference's constants."""
+    Q = _empty(p)
+    X, Y, Z, _ = _grid(p)
+    xr = 0.5 * (p.xmin + p.xmax) - X
+    yr = 0.5 * (p.ymin + p.ymax) - Y
+    zr = 0.5 * (p.zmin + p.zmax) - Z
+    r = np.sqrt(xr * xr + yr * yr + zr * zr)
+    inside = r < radius
+    d = _dom(Q, p)
+    d[..., 0] = np.where(inside, rho_in, rho_out)
+    d[..., 4] = np.where(inside, p_in, p_out)
+    return Q
+
+
+def diffusion(p):
+    """initDiffusion (Init.h:155-178)."""
+    Q = _empty(p)
+    X, Y, Z, _ = _grid(p)
+    x0 = X - 0.5 * (p.xmin + p.xmax)
+    y0 = Y - 0.5 * (p.ymin + p.ymax)
+    z0 = Z - 0.5 * (p.zmin + p.zmax)
+    r = np.sqrt(x0 * x0 + y0 * y0 + z0 * z0)
+    d = _dom(Q, p)
+    d[..., 0] = np.where(r < 0.2, 1.0, 0.1)
+    d[..., 1:5] = 1.0
+    return Q
+
+
+def rayleigh_taylor(p):
+    """initRayleighTaylor (Init.h:183-209)."""
+    Q = _empty(p)
+    X, Y, Z, _ = _grid(p)
+    zmid = 0.5 * (p.zmin + p.zmax)
+    d = _dom(Q, p)
+    d[..., 0] = np.where(Z < zmid, 1.0, 2.0)
+    d[..., 4] = 2.5 + 0.1 * p.g * Z
+    band = (Z > -1.0 / 3.0) & (Z < 1.0 / 3.0)
+    w = 0.01 * (1.0 + np.cos(4 * np.pi * X)) * (1.0 + np.cos(4.0 * np.pi * Y)) * \
+        (1 + np.cos(3.0 * np.pi * Z)) / 4.0
+    d[..., 3] = np.where(band, w, 0.0)
+    return Q
+
+
+def h84(p):
+    """initH84 (Init.h:104-123) with the counter-based perturbation."""
+    Q = _empty(p)
+    X, Y, Z, cells = _grid(p)
+    d = _dom(Q, p)
+    d[..., 0] = np.power(Z, p.m1)
+    d[..., 3] = p.h84_pert * cell_uniform(p.seed, cells, -0.5, 0.5)
+    d[..., 4] = np.power(Z, p.m1 + 1.0)
+    return Q
+
+
+def c91(p):
+    """initC91 (Init.h:128-150) with the counter-based perturbation."""
+    Q = _empty(p)
+    X, Y, Z, cells = _grid(p)
+    T = 1.0 + p.theta1 * Z
+    d = _dom(Q, p)
+    d[..., 0] = np.power(T, p.m1)
+    prs = np.power(T, p.m1 + 1.0)
+    d[..., 4] = prs * (1.0 + p.c91_pert * cell_uniform(p.seed, cells, -0.5, 0.5))
+    return Q
+
+
+def polytrope_convection(p, m=1.0, amplitude=1.0e-3):
+    """Stratified polytropic layer with velocity noise (the 256x256x128
+    conduction/viscosity workload).  z is depth: T = 1 + theta*z with
+    theta = g/(m+1) so that dp/dz = rho*g (source +rho*g, Update.h:149);
+    rho = T^m, p = T^(m+1); u, v, w += amplitude*U(-1/2, 1/2), seeded."""
+    Q = _empty(p)
+    X, Y, Z, cells = _grid(p)
+    theta = p.g / (m + 1.0)
+    T = 1.0 + theta * Z
+    d = _dom(Q, p)
+    d[..., 0] = np.power(T, m)
+    d[..., 4] = np.power(T, m + 1.0)
+    for s, slot in enumerate((1, 2, 3)):
+        d[..., slot] = amplitude * cell_uniform(p.seed, cells, -0.5, 0.5, stream=s + 1)
+    return Q
+
+
+def initial_condition(cfg) -> np.ndarray:
+    """Q for a RunConfig (read_inifile) -- InitFunctor::init (Init.h:253-278)."""
+    p = cfg.params
+    name = cfg.problem
+    if name == "sod_x":
+        return sod(p, 0)
+    if name == "sod_y":
+        return sod(p, 1)
+    if name == "sod_z":
+        return sod(p, 2)
+    if name == "blast":
+        return blast(p, cfg.blast_radius, cfg.blast_rho_in, cfg.blast_p_in, cfg.blast_rho_out,
+                     cfg.blast_p_out)
+    if name == "diffusion":
+        return diffusion(p)
+    if name == "rayleigh-taylor":
+        return rayleigh_taylor(p)
+    if name == "H84":
+        return h84(p)
+    if name == "C91":
+        return c91(p)
+    if name == "polytrope":
+        return polytrope_convection(p)
+    raise RuntimeError("Error unknown problem " + name)  # Init.h:246-247

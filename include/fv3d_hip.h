@@ -1,0 +1,103 @@
+/*
+ * fv3d_hip.h -- C ABI of libfv3d_hip, the MI355X (gfx950) implementation of
+ * fv3d's per-timestep finite-volume hydro path.
+ *
+ * Conventions
+ *  - Every entry point returns 0 on success and a non-zero FV3D_E* code on
+ *    failure; fv3d_last_error(ctx) (or fv3d_last_error(NULL) for calls that
+ *    take no context) gives a message.  Nothing throws across the ABI.
+ *  - A context binds one device, one HIP stream and all device buffers of one
+ *    box.  Calls on a context are stream-ordered and not thread-safe.
+ *  - Host arrays use the reference's host layout: (Ntz, Nty, Ntx, 5) with
+ *    ghosts, or (Nz, Ny, Nx, 5) without, variable fastest
+ *    (Kokkos::View<double****> LayoutRight, SimInfo.h:14, States.h:5-18).
+ *  - On the device the state is conserved variables in SoA layout
+ *    [5][Nz][Ny][Nx] (domain only, x fastest); ghosts are produced inside the
+ *    kernels from the boundary index maps.
+ */
+#ifndef FV3D_HIP_H
+#define FV3D_HIP_H
+
+#include <stddef.h>
+#include "fv3d_params.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FV3D_OK 0
+#define FV3D_EINVAL 1      /* bad argument or unsupported parameter combination */
+#define FV3D_EHIP 2        /* a HIP runtime call failed */
+#define FV3D_ENOMEM 3      /* device allocation failed */
+#define FV3D_ENONFINITE 4  /* the state produced NaN/inf (pressure < 0, density 0, ...) */
+
+#define FV3D_FP64 0
+#define FV3D_FP32 1
+
+typedef struct fv3d_ctx fv3d_ctx;
+
+/* library identification: ABI version and sizeof(fv3d_params) it was built with */
+int fv3d_abi_version(void);
+size_t fv3d_params_size(void);
+const char* fv3d_last_error(const fv3d_ctx* ctx);
+
+/* Context = the state owned by the reference's main() (main.cpp:28-30) plus
+ * the functors built from Params (main.cpp:38-41): UpdateFunctor
+ * (Update.h:45), ComputeDtFunctor (ComputeDt.h:13), BoundaryManager
+ * (BoundaryConditions.h:150).  precision is FV3D_FP64 or FV3D_FP32. */
+int fv3d_create(const fv3d_params* p, int precision, int device, fv3d_ctx** out);
+int fv3d_destroy(fv3d_ctx* ctx);
+
+/* primToCons(Q, U) after init (main.cpp:43-44, SimInfo.h:323-331): Q is a
+ * host primitive array, with ghosts if with_ghosts (ghost values are not
+ * read).  Resets the step counter and computes the first inv_dt_hyp. */
+int fv3d_set_prim(fv3d_ctx* ctx, const double* Q, int with_ghosts);
+/* consToPrim(U, Q) (SimInfo.h:313-321) of the current state; with ghosts the
+ * ghost shell is filled by fillBoundaries (BoundaryConditions.h:146-215),
+ * which is what the reference's Q holds inside update(). */
+int fv3d_get_prim(fv3d_ctx* ctx, double* Q, int with_ghosts);
+/* The reference's Q after update() returns (main.cpp:76 snapshot source):
+ * RK2: prim(U*) of the last step (Update.h:194); Euler: prim(U^n) of the
+ * last step.  Ghosts filled as above. */
+int fv3d_get_stage_prim(fv3d_ctx* ctx, double* Q, int with_ghosts);
+/* conserved state in host layout (checkpoint / restart). */
+int fv3d_set_cons(fv3d_ctx* ctx, const double* U, int with_ghosts);
+int fv3d_get_cons(fv3d_ctx* ctx, double* U, int with_ghosts);
+
+/* ComputeDtFunctor::computeDt (ComputeDt.h:17-63) on prim(U) of the current
+ * state: inv[3] = {inv_dt_hyp, inv_dt_par_tc, inv_dt_par_visc}, *dt =
+ * CFL/max(inv).  Synchronises; returns FV3D_ENONFINITE if a cell produced a
+ * non-finite inverse time step (the reference silently drops such cells). */
+int fv3d_compute_dt(fv3d_ctx* ctx, double inv[3], double* dt);
+
+/* UpdateFunctor::update (Update.h:180-206) with dt = computeDt(prim(U)):
+ * one Euler or SSP-RK2 step per params.time_stepping.  dt stays on the
+ * device (no host synchronisation); read it with fv3d_compute_dt before the
+ * step as main.cpp:57 does. */
+int fv3d_step(fv3d_ctx* ctx);
+/* nsteps steps back to back (no host synchronisation in between). */
+int fv3d_steps(fv3d_ctx* ctx, int nsteps);
+/* The number of steps taken since fv3d_set_prim / fv3d_set_cons. */
+long fv3d_step_count(const fv3d_ctx* ctx);
+int fv3d_synchronize(fv3d_ctx* ctx);
+
+/* Timing on the context's stream with HIP events: runs nsteps steps and
+ * returns the elapsed milliseconds of the whole window and of each stage
+ * kernel launch (stage_ms must hold 2*nsteps floats for RK2, nsteps for
+ * Euler; may be NULL). */
+int fv3d_time_steps(fv3d_ctx* ctx, int nsteps, float* total_ms, float* stage_ms);
+
+/* ---- functor-level entry points on host arrays (unit parity) ---- */
+
+/* BoundaryManager::fillBoundaries (BoundaryConditions.h:146-215) on a host
+ * primitive array with ghosts, run on the device. */
+int fv3d_fill_boundaries(const fv3d_params* p, double* Q);
+/* hll / hllc (RiemannSolvers.h:6,39) on n face-frame state pairs [n][5]. */
+int fv3d_riemann(int solver, double gamma0, long n, const double* qL, const double* qR,
+                 double* flux, double* pout);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FV3D_HIP_H */
