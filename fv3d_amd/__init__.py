@@ -165,6 +165,7 @@ def lib() -> ctypes.CDLL:
         "fv3d_time_steps": (ctypes.c_int, [vp, ctypes.c_int, P(ctypes.c_float), P(ctypes.c_float)]),
         "fv3d_fill_boundaries": (ctypes.c_int, [P(Params), dp]),
         "fv3d_riemann": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_long, dp, dp, dp, dp]),
+        "fv3d_stream_copy": (ctypes.c_int, [ctypes.c_size_t, ctypes.c_int, P(ctypes.c_float)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -293,6 +294,13 @@ def fill_boundaries(params: Params, Q: np.ndarray) -> np.ndarray:
         raise ValueError("Q must include ghosts")
     _check(lib().fv3d_fill_boundaries(ctypes.byref(params), _dptr(Q)))
     return Q
+
+
+def stream_copy_gbs(nbytes: int = 1 << 30, reps: int = 20) -> float:
+    """Measured device copy bandwidth (GB/s, read + write bytes)."""
+    ms = ctypes.c_float()
+    _check(lib().fv3d_stream_copy(int(nbytes), int(reps), ctypes.byref(ms)))
+    return 2.0 * nbytes / (ms.value * 1e-3) / 1e9
 
 
 def riemann(solver: int, gamma0: float, qL: np.ndarray, qR: np.ndarray):

@@ -470,6 +470,36 @@ int fv3d_fill_boundaries(const fv3d_params* p, double* Q) {
   return FV3D_OK;
 }
 
+int fv3d_stream_copy(size_t nbytes, int reps, float* ms_per_copy) {
+  if (nbytes < 16 || reps <= 0) return fail(nullptr, FV3D_EINVAL, "bad argument");
+  const long long n = (long long)(nbytes / 16) * 2;
+  double *a = nullptr, *b = nullptr;
+  hipStream_t s;
+  hipEvent_t e0, e1;
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess)
+    return fail(nullptr, FV3D_EHIP, "hipStreamCreate failed");
+  hipError_t e = hipMalloc(&a, n * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&b, n * sizeof(double));
+  if (e == hipSuccess) e = hipMemsetAsync(a, 0, n * sizeof(double), s);
+  if (e == hipSuccess) e = launch_copy(a, b, n, s);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = hipEventRecord(e0, s);
+  for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch_copy(a, b, n, s);
+  if (e == hipSuccess) e = hipEventRecord(e1, s);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float ms = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  if (ms_per_copy) *ms_per_copy = ms / reps;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (a) hipFree(a);
+  if (b) hipFree(b);
+  hipStreamDestroy(s);
+  if (e != hipSuccess) return fail(nullptr, FV3D_EHIP, std::string("stream_copy: ") + hipGetErrorString(e));
+  return FV3D_OK;
+}
+
 int fv3d_riemann(int solver, double gamma0, long n, const double* qL, const double* qR,
                  double* flux, double* pout) {
   if (n < 0 || (n > 0 && (!qL || !qR || !flux || !pout)))

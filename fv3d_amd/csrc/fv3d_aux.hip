@@ -264,6 +264,21 @@ __global__ void k_riemann(int solver, double gamma, long n, const double* __rest
   pout[s] = po;
 }
 
+// plain copy of n doubles (16-byte accesses): the STREAM-copy reference
+// kernel, and the calibration kernel for the profiler's byte counters
+__global__ void __launch_bounds__(256) k_copy(const double2* __restrict__ a, double2* __restrict__ b,
+                                              long long n2) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n2;
+       i += (long long)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+
+hipError_t launch_copy(const double* a, double* b, long long n, hipStream_t s) {
+  const long long n2 = n / 2;
+  hipLaunchKernelGGL(k_copy, dim3(8192), dim3(256), 0, s, (const double2*)a, (double2*)b, n2);
+  return hipGetLastError();
+}
+
 __global__ void k_zero_u64(unsigned long long* p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0ull;

@@ -44,7 +44,8 @@ hipError_t launch_dt_reduce(const real* U, unsigned long long* inv, int* health,
 // batched Riemann solves over [n][5] states (face frame)
 hipError_t launch_riemann_batch(int solver, double gamma, long n, const double* qL,
                                 const double* qR, double* flux, double* pout, hipStream_t s);
-// copy of n elements with a cast (state conversions between precisions)
+// plain device copy of n doubles (n even)
+hipError_t launch_copy(const double* a, double* b, long long n, hipStream_t s);
 hipError_t launch_fill_zero_u64(unsigned long long* p, int n, hipStream_t s);
 
 }  // namespace fv3d_hip

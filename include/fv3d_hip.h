@@ -96,6 +96,12 @@ int fv3d_fill_boundaries(const fv3d_params* p, double* Q);
 int fv3d_riemann(int solver, double gamma0, long n, const double* qL, const double* qR,
                  double* flux, double* pout);
 
+/* Measurement support (no counterpart in the reference): a plain device copy
+ * of nbytes (16-byte accesses), run reps times; *ms_per_copy is the mean
+ * time of one copy from HIP events.  Gives the measured copy bandwidth and
+ * calibrates the profiler's byte counters (nbytes read + nbytes written). */
+int fv3d_stream_copy(size_t nbytes, int reps, float* ms_per_copy);
+
 #ifdef __cplusplus
 }
 #endif
