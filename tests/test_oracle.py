@@ -1,0 +1,433 @@
+"""Pins for the oracle (the CPU restatement of the reference) that do not
+come from the oracle itself: closed forms, textbook solvers written in a
+different form, published known answers, invariants and brute force.
+
+The reference ships no tests, golden vectors or fixtures, and it cannot be
+built here (Kokkos, HighFive and inih are not vendored), so these checks
+are what stand between a transcription mistake in the oracle and the GPU
+parity tests that compare against it.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import fv3d_amd as F
+from fv3d_amd import Params
+from oracle.oracle import Oracle
+from tests.exact_riemann import sample, star_state
+from tests.helpers import sod_c1, dom
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def rel(a, b, floor=1.0):
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b)) / np.maximum(np.abs(b), floor)))
+
+
+# ---------------------------------------------------------------------------
+# exact Riemann solver against the published table
+# ---------------------------------------------------------------------------
+def test_exact_solver_matches_toro_table():
+    g = json.load(open(os.path.join(HERE, "golden", "sod_toro_test1.json")))
+    L, R = g["left"], g["right"]
+    ps, us = star_state(L["rho"], L["u"], L["p"], R["rho"], R["u"], R["p"], g["gamma"])
+    assert round(ps, 5) == g["p_star"]
+    assert round(us, 5) == g["u_star"]
+    s = sample(L["rho"], L["u"], L["p"], R["rho"], R["u"], R["p"], g["gamma"],
+               np.array([us - 1e-3, us + 1e-3]))
+    assert round(s[0, 0], 5) == g["rho_star_left"]
+    assert round(s[1, 0], 5) == g["rho_star_right"]
+
+
+# ---------------------------------------------------------------------------
+# state algebra
+# ---------------------------------------------------------------------------
+def test_prim_cons_closed_form_and_round_trip():
+    p = Params.defaults().replace(gamma0=1.4)
+    o = Oracle(p)
+    Q = o.zeros()
+    Q[2, 2, 2] = [2.0, 1.0, -1.0, 0.5, 3.0]
+    U = o.prim_to_cons(Q)
+    # E = 0.5*rho*|v|^2 + p/(gamma-1) = 0.5*2*2.25 + 3/0.4
+    assert U[2, 2, 2].tolist() == pytest.approx([2.0, 2.0, -2.0, 1.0, 2.25 + 7.5], rel=1e-15)
+    rng = np.random.default_rng(0)
+    Qr = o.zeros()
+    Qr[..., 0] = rng.uniform(0.1, 3, Qr.shape[:3])
+    Qr[..., 1:4] = rng.normal(0, 2, Qr.shape[:3] + (3,))
+    Qr[..., 4] = rng.uniform(0.1, 3, Qr.shape[:3])
+    back = o.cons_to_prim(o.prim_to_cons(Qr))
+    assert rel(back, Qr, floor=1.0) < 1e-13
+
+
+def _euler_flux(q, g):
+    """Textbook x-flux of the Euler equations from primitive variables."""
+    r, u, v, w, p = q.T
+    E = p / (g - 1.0) + 0.5 * r * (u * u + v * v + w * w)
+    return np.stack([r * u, r * u * u + p, r * u * v, r * u * w, u * (E + p)], axis=1)
+
+
+def _cons(q, g):
+    r, u, v, w, p = q.T
+    return np.stack([r, r * u, r * v, r * w, p / (g - 1.0) + 0.5 * r * (u * u + v * v + w * w)], axis=1)
+
+
+def _states(n, seed, w_zero=False):
+    rng = np.random.default_rng(seed)
+    q = np.empty((n, 5))
+    q[:, 0] = rng.uniform(0.1, 3.0, n)
+    q[:, 1:4] = rng.normal(0.0, 1.0, (n, 3))
+    q[:, 4] = rng.uniform(0.05, 4.0, n)
+    if w_zero:
+        q[:, 3] = 0.0
+    return q
+
+
+# ---------------------------------------------------------------------------
+# HLL (RiemannSolvers.h:5-36)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("g", [1.4, 5.0 / 3.0])
+def test_hll_consistency_and_supersonic(g):
+    p = Params.defaults().replace(gamma0=g)
+    o = Oracle(p)
+    q = _states(2000, 1)
+    f, po = o.riemann(F.HLL, q, q)
+    assert rel(f, _euler_flux(q, g)) < 1e-12
+    assert np.array_equal(po, q[:, 4])
+    # supersonic to the right: every wave moves right, flux = left flux exactly
+    qL, qR = _states(500, 2), _states(500, 3)
+    qL[:, 1] += 30.0
+    qR[:, 1] += 30.0
+    f, po = o.riemann(F.HLL, qL, qR)
+    assert rel(f, _euler_flux(qL, g)) < 1e-14 and np.array_equal(po, qL[:, 4])
+    qL[:, 1] -= 60.0
+    qR[:, 1] -= 60.0
+    f, po = o.riemann(F.HLL, qL, qR)
+    assert rel(f, _euler_flux(qR, g)) < 1e-14 and np.array_equal(po, qR[:, 4])
+
+
+@pytest.mark.parametrize("g", [1.4, 5.0 / 3.0])
+def test_hll_rankine_hugoniot(g):
+    """In the subsonic case the HLL flux is the flux of the single HLL
+    intermediate state: F = FL + SL(U*-UL) = FR + SR(U*-UR) with
+    U* = (SR UR - SL UL - (FR - FL))/(SR - SL) and Davis speeds."""
+    p = Params.defaults().replace(gamma0=g)
+    o = Oracle(p)
+    qL, qR = _states(3000, 4), _states(3000, 5)
+    f, po = o.riemann(F.HLL, qL, qR)
+    cL = np.sqrt(g * qL[:, 4] / qL[:, 0])
+    cR = np.sqrt(g * qR[:, 4] / qR[:, 0])
+    SL = np.minimum(qL[:, 1] - cL, qR[:, 1] - cR)[:, None]
+    SR = np.maximum(qL[:, 1] + cL, qR[:, 1] + cR)[:, None]
+    sub = ((SL < 0) & (SR > 0))[:, 0]
+    assert sub.sum() > 1000
+    FL, FR, UL, UR = _euler_flux(qL, g), _euler_flux(qR, g), _cons(qL, g), _cons(qR, g)
+    Us = (SR * UR - SL * UL - (FR - FL)) / (SR - SL)
+    scale = np.max(np.abs(np.concatenate([FL, FR, SL * UL, SR * UR], axis=1)), axis=1, keepdims=True)
+    err_l = np.abs(f - (FL + SL * (Us - UL))) / scale
+    err_r = np.abs(f - (FR + SR * (Us - UR))) / scale
+    assert err_l[sub].max() < 1e-12 and err_r[sub].max() < 1e-12
+    assert np.array_equal(po[sub], 0.5 * (qL[sub, 4] + qR[sub, 4]))
+
+
+# ---------------------------------------------------------------------------
+# HLLC (RiemannSolvers.h:38-112)
+# ---------------------------------------------------------------------------
+def _toro_hllc(qL, qR, g):
+    """HLLC in Toro's form (ch. 10.4): S* from the jump conditions, star
+    states U*K and F*K = FK + SK (U*K - UK); wave speeds as the reference
+    estimates them, SL = min(uL,uR) - max(cL,cR), SR = max(uL,uR) + max(cL,cR)."""
+    rL, uL, vL, wL, pL = qL.T
+    rR, uR, vR, wR, pR = qR.T
+    cL, cR = np.sqrt(g * pL / rL), np.sqrt(g * pR / rR)
+    SL = np.minimum(uL, uR) - np.maximum(cL, cR)
+    SR = np.maximum(uL, uR) + np.maximum(cL, cR)
+    Ss = (pR - pL + rL * uL * (SL - uL) - rR * uR * (SR - uR)) / (rL * (SL - uL) - rR * (SR - uR))
+    FL, FR, UL, UR = _euler_flux(qL, g), _euler_flux(qR, g), _cons(qL, g), _cons(qR, g)
+
+    def ustar(r, u, v, w, p, U, S):
+        fac = r * (S - u) / (S - Ss)
+        E = U[:, 4]
+        return np.stack([fac, fac * Ss, fac * v, fac * w,
+                         fac * (E / r + (Ss - u) * (Ss + p / (r * (S - u))))], axis=1)
+
+    FsL = FL + SL[:, None] * (ustar(rL, uL, vL, wL, pL, UL, SL) - UL)
+    FsR = FR + SR[:, None] * (ustar(rR, uR, vR, wR, pR, UR, SR) - UR)
+    out = np.where((SL > 0)[:, None], FL,
+                   np.where((Ss > 0)[:, None], FsL, np.where((SR > 0)[:, None], FsR, FR)))
+    return out
+
+
+@pytest.mark.parametrize("g", [1.4, 5.0 / 3.0])
+def test_hllc_against_textbook_form(g):
+    """With w = 0 the reference's kinetic energy (which drops the second
+    transverse velocity) is complete, and its HLLC must agree with Toro's
+    formulation of the same solver."""
+    p = Params.defaults().replace(gamma0=g)
+    o = Oracle(p)
+    qL, qR = _states(4000, 6, w_zero=True), _states(4000, 7, w_zero=True)
+    qL[:500, 1] += 25.0
+    qR[:500, 1] += 25.0
+    qL[500:1000, 1] -= 25.0
+    qR[500:1000, 1] -= 25.0
+    f, _ = o.riemann(F.HLLC, qL, qR)
+    ref = _toro_hllc(qL, qR, g)
+    scale = np.maximum(np.max(np.abs(ref), axis=1, keepdims=True), 1.0)
+    assert np.max(np.abs(f - ref) / scale) < 1e-12
+
+
+def test_hllc_consistency_and_dropped_velocity():
+    """qL == qR gives the Euler flux when w = 0.  With w != 0 the energy flux
+    lacks exactly 0.5*rho*w^2*u (RiemannSolvers.h:52-55)."""
+    g = 1.4
+    o = Oracle(Params.defaults().replace(gamma0=g))
+    q = _states(1000, 8, w_zero=True)
+    f, po = o.riemann(F.HLLC, q, q)
+    # pout is the star pressure, which equals p up to rounding
+    assert rel(f, _euler_flux(q, g)) < 1e-12 and np.allclose(po, q[:, 4], rtol=1e-14)
+    q = _states(1000, 9)
+    f, _ = o.riemann(F.HLLC, q, q)
+    ex = _euler_flux(q, g)
+    ex[:, 4] -= 0.5 * q[:, 0] * q[:, 3] ** 2 * q[:, 1]
+    assert rel(f, ex) < 1e-12
+
+
+# ---------------------------------------------------------------------------
+# slopes (Update.h:55-88)
+# ---------------------------------------------------------------------------
+def test_minmod_slopes_hand_cases():
+    p = Params.defaults().replace(Nx=6, Ny=3, Nz=3)
+    o = Oracle(p)
+    Q = o.zeros() + 1.0
+    row = np.array([0.0, 1.0, 3.0, 2.0, 2.0, 5.0, 4.0, 4.5, 7.0, 7.0])  # i = 0..9
+    Q[:, :, :, 0] = row[None, None, :]
+    sx, sy, sz = o.compute_slopes(Q)
+    # slope at i: minmod(Q[i]-Q[i-1], Q[i+1]-Q[i]) for i in [ibeg-1, iend] = [1, 8]
+    expect = {1: 1.0, 2: 0.0, 3: 0.0, 4: 0.0, 5: 0.0, 6: 0.0, 7: 0.5, 8: 0.0}
+    for i, v in expect.items():
+        assert sx[3, 3, i, 0] == v, i
+    assert np.all(sy[2:-2, 2:-2, 1:-1, 0] == 0) and np.all(sz[2:-2, 2:-2, 1:-1, 0] == 0)
+    # equal magnitudes: x takes the forward difference, z the backward one
+    Q = o.zeros() + 1.0
+    Q[:, :, :, 1] = np.array([0.0, 0.0, 1.0, 2.0, 3.0, 4.0, 5.0, 6.0, 7.0, 8.0])[None, None, :]
+    Q[:, :, :, 2] = np.array([0.0, 0.0, -1.0, -2.0, -3.0, -4.0, -5.0])[:, None, None]
+    sx, sy, sz = o.compute_slopes(Q)
+    assert sx[3, 3, 4, 1] == 1.0 and sz[3, 3, 4, 2] == -1.0
+
+
+# ---------------------------------------------------------------------------
+# ghost fill (BoundaryConditions.h:146-215) by brute force
+# ---------------------------------------------------------------------------
+def _map(i, beg, end, N, bc):
+    """Source index along one axis and whether the velocity is negated."""
+    if beg <= i < end:
+        return i, False
+    if bc == F.BC_ABSORBING:
+        return (beg if i < beg else end - 1), False
+    if bc == F.BC_REFLECTING:
+        return (2 * beg - i - 1 if i < beg else 2 * end - i - 1), True
+    return (i + N if i < beg else i - N), False
+
+
+@pytest.mark.parametrize("bcs", [(a, b, c) for a in (0, 1, 2) for b in (0, 1, 2) for c in (0, 1, 2)])
+def test_ghost_fill_is_separable_map(bcs):
+    p = Params.defaults().replace(Nx=5, Ny=4, Nz=3, boundary_x=bcs[0], boundary_y=bcs[1], boundary_z=bcs[2])
+    o = Oracle(p)
+    rng = np.random.default_rng(sum(bcs))
+    Q = rng.uniform(1, 2, p.shape_ghosted)
+    src = Q.copy()
+    o.fill_boundaries(Q)
+    for k in range(p.Ntz):
+        for j in range(p.Nty):
+            for i in range(p.Ntx):
+                si, fx = _map(i, p.ibeg, p.iend, p.Nx, bcs[0])
+                sj, fy = _map(j, p.jbeg, p.jend, p.Ny, bcs[1])
+                sk, fz = _map(k, p.kbeg, p.kend, p.Nz, bcs[2])
+                v = src[sk, sj, si].copy()
+                if fx: v[1] = -v[1]
+                if fy: v[2] = -v[2]
+                if fz: v[3] = -v[3]
+                assert np.array_equal(Q[k, j, i], v), (i, j, k)
+
+
+# ---------------------------------------------------------------------------
+# dt (ComputeDt.h) closed forms at t = 0
+# ---------------------------------------------------------------------------
+def test_first_dt_closed_forms():
+    p = sod_c1()
+    o = Oracle(p)
+    Q = o.cons_to_prim(o.prim_to_cons(o.init("sod_x")))
+    dt, inv = o.compute_dt(Q)
+    assert dt == pytest.approx(0.8 / (math.sqrt(1.4) * (256 + 4 + 4)), rel=1e-15)
+    assert inv[1] == p.epsilon and inv[2] == p.epsilon
+    p = Params.defaults().replace(Nx=64, Ny=64, Nz=64, CFL=0.8)
+    o = Oracle(p)
+    Q = F.problems.blast(p, radius=0.1, rho_in=1.0, p_in=10.0, rho_out=1.0, p_out=0.1)
+    dt, _ = o.compute_dt(Q)
+    assert dt == pytest.approx(0.8 / (3 * math.sqrt(10.0 * (5.0 / 3.0)) * 64), rel=1e-15)
+    p = p.replace(thermal_conductivity_active=1, kappa=0.07, viscosity_active=1, mu=0.005)
+    dt, inv = Oracle(p).compute_dt(Q)
+    assert inv[1] == pytest.approx(2 * 0.07 * 64 * 64, rel=1e-15)
+    assert inv[2] == pytest.approx(2 * 0.005 * 64 * 64, rel=1e-15)
+
+
+# ---------------------------------------------------------------------------
+# whole scheme: exact Sod solution, conservation, symmetry
+# ---------------------------------------------------------------------------
+def _sod_l1(N):
+    p = sod_c1(Nx=N)
+    o = Oracle(p)
+    U, _, dts, t = o.run(o.init("sod_x"), 100000, stop_at_tend=True)
+    q = dom(p, o.cons_to_prim(U))[0, 0]
+    x = p.xmin + (np.arange(N) + 0.5) * p.dx
+    ex = sample(1.0, 0.0, 1.0, 0.125, 0.0, 0.1, 1.4, (x - 0.5) / t)
+    return np.mean(np.abs(q[:, 0] - ex[:, 0])), q, ex, t
+
+
+def test_sod_against_exact_solution_and_converges():
+    e256, q, ex, t = _sod_l1(256)
+    assert e256 < 5e-3
+    # the star plateaux of the table: rarefaction tail just left of x = 0.5,
+    # contact at 0.5 + u* t, shock at about 0.5 + 1.75 t
+    x = (np.arange(256) + 0.5) / 256
+    us = 0.92745
+    left_plateau = (x > 0.5) & (x < 0.5 + 0.5 * us * t)
+    right_plateau = (x > 0.5 + 1.15 * us * t) & (x < 0.5 + 1.6 * us * t)
+    assert np.allclose(q[left_plateau, 0], 0.42632, rtol=1e-2)
+    assert np.allclose(q[right_plateau, 0], 0.26557, rtol=1e-2)
+    assert np.allclose(q[left_plateau | right_plateau, 4], 0.30313, rtol=1e-2)
+    e128, _, _, _ = _sod_l1(128)
+    e64, _, _, _ = _sod_l1(64)
+    assert e256 < 0.8 * e128 < 0.64 * e64
+
+
+def _smooth(p, seed):
+    rng = np.random.default_rng(seed)
+    Q = np.zeros(p.shape_ghosted)
+    d = dom(p, Q)
+    z, y, x = np.meshgrid(np.arange(p.Nz), np.arange(p.Ny), np.arange(p.Nx), indexing="ij")
+    d[..., 0] = 1.0 + 0.3 * np.sin(2 * np.pi * x / p.Nx) * np.cos(2 * np.pi * y / p.Ny)
+    d[..., 1] = 0.4 * np.sin(2 * np.pi * z / p.Nz) + 0.05 * rng.standard_normal(x.shape)
+    d[..., 2] = 0.3 * np.cos(2 * np.pi * x / p.Nx)
+    d[..., 3] = 0.2 * np.sin(2 * np.pi * y / p.Ny)
+    d[..., 4] = 1.0 + 0.2 * rng.random(x.shape)
+    return Q
+
+
+@pytest.mark.parametrize("solver", [F.HLL, F.HLLC])
+@pytest.mark.parametrize("recon", [F.PCM, F.PLM])
+@pytest.mark.parametrize("ts", [F.TS_EULER, F.TS_RK2])
+def test_conservation_periodic(solver, recon, ts):
+    """Mass, momentum and energy to round-off with periodic boundaries
+    (hydro and conduction; viscosity's z-branch is not conservative)."""
+    p = Params.defaults().replace(Nx=12, Ny=10, Nz=8, boundary_x=F.BC_PERIODIC, boundary_y=F.BC_PERIODIC,
+                                  boundary_z=F.BC_PERIODIC, riemann_solver=solver, reconstruction=recon,
+                                  time_stepping=ts, thermal_conductivity_active=1, kappa=0.01, tend=1e9)
+    o = Oracle(p)
+    Q0 = _smooth(p, 11)
+    U0 = o.prim_to_cons(Q0)
+    U, _, _, _ = o.run(Q0, 15, stop_at_tend=False)
+    s0 = dom(p, U0).reshape(-1, 5).sum(axis=0)
+    s1 = dom(p, U).reshape(-1, 5).sum(axis=0)
+    scale = dom(p, np.abs(U0)).reshape(-1, 5).sum(axis=0)
+    assert np.max(np.abs(s1 - s0) / scale) < 1e-13
+
+
+def test_blast_mirror_symmetry():
+    p = Params.defaults().replace(Nx=16, Ny=16, Nz=16, reconstruction=F.PLM, time_stepping=F.TS_RK2)
+    o = Oracle(p)
+    U, _, _, _ = o.run(F.problems.blast(p), 12, stop_at_tend=False)
+    q = dom(p, o.cons_to_prim(U))
+    m = q[:, :, ::-1].copy()
+    m[..., 1] = -m[..., 1]
+    assert np.array_equal(q, m)
+    m = q[::-1].copy()
+    m[..., 3] = -m[..., 3]
+    assert np.array_equal(q, m)
+
+
+# ---------------------------------------------------------------------------
+# conduction (ThermalConduction.h) and viscosity (Viscosity.h)
+# ---------------------------------------------------------------------------
+def test_conduction_linear_profile_and_mode_decay():
+    p = Params.defaults().replace(Nx=16, Ny=4, Nz=4, boundary_x=F.BC_PERIODIC, boundary_y=F.BC_PERIODIC,
+                                  boundary_z=F.BC_PERIODIC, thermal_conductivity_active=1, kappa=0.3)
+    o = Oracle(p)
+    x = (np.arange(p.Ntx) - p.ibeg + 0.5) * p.dx
+    k = 2 * np.pi
+    a = 0.1
+    Q = o.zeros()
+    Q[..., 0] = 2.0
+    Q[..., 4] = 2.0 * (1.0 + a * np.sin(k * x))[None, None, :]  # T = 1 + a sin(kx)
+    o.fill_boundaries(Q)
+    Unew = np.zeros_like(Q)
+    dt = 1e-3
+    o.thermal_conduction(Q, Unew, dt)
+    # discrete Laplacian eigenvalue of the 3-point stencil
+    lam = (2 * np.cos(k * p.dx) - 2) / p.dx ** 2
+    expect = dt * 0.3 * lam * a * np.sin(k * x[p.ibeg:p.iend])
+    assert np.allclose(dom(p, Unew)[1, 1, :, 4], expect, rtol=1e-9, atol=1e-15)
+    # linear temperature: no interior divergence
+    p2 = p.replace(boundary_x=F.BC_ABSORBING)
+    o2 = Oracle(p2)
+    Q = o2.zeros()
+    Q[..., 0] = 1.0
+    Q[..., 4] = (1.0 + 0.5 * x)[None, None, :]
+    Unew = np.zeros_like(Q)
+    o2.thermal_conduction(Q, Unew, dt)
+    assert np.max(np.abs(dom(p2, Unew)[:, :, 1:-1, 4])) < 1e-13
+
+
+def test_conduction_fixed_temperature_wall():
+    p = Params.defaults().replace(Nx=4, Ny=4, Nz=6, boundary_x=F.BC_PERIODIC, boundary_y=F.BC_PERIODIC,
+                                  boundary_z=F.BC_REFLECTING, thermal_conductivity_active=1, kappa=0.5,
+                                  bctc_zmin=F.BCTC_FIXED_TEMPERATURE, bctc_zmin_value=2.0,
+                                  bctc_zmax=F.BCTC_FIXED_GRADIENT, bctc_zmax_value=0.25)
+    o = Oracle(p)
+    Q = o.zeros()
+    Q[..., 0] = 1.0
+    Q[..., 4] = 1.0
+    o.fill_boundaries(Q)
+    Unew = np.zeros_like(Q)
+    dt = 1e-2
+    o.thermal_conduction(Q, Unew, dt)
+    dE = dom(p, Unew)[..., 4]
+    # uniform T=1: only the walls conduct; bottom: F_F = 2*kappa*(T - Tb)/dz,
+    # top: F_B = kappa*gradient; E += dt/dz*(F_B - F_F)
+    assert np.allclose(dE[0], dt / p.dz * (0.0 - 2 * 0.5 * (1.0 - 2.0) / p.dz), rtol=1e-14)
+    assert np.allclose(dE[-1], dt / p.dz * (0.5 * 0.25 - 0.0), rtol=1e-14)
+    assert np.all(dE[1:-1] == 0.0)
+
+
+def test_viscosity_shear_closed_forms():
+    """u = sin(k y) only.  x momentum: dt*mu*(u[j+1]-2u[j]+u[j-1])/dy (the
+    viscous update has no 1/d factor, Viscosity.h:151); y momentum: 0; z
+    momentum from the z-branch cross term as written (Viscosity.h:122-133):
+    dt*mu*(-2/3)*(0.25/dx)*(u[j-1]-u[j+1]).  Uniform flow: no update."""
+    p = Params.defaults().replace(Nx=4, Ny=16, Nz=4, boundary_x=F.BC_PERIODIC, boundary_y=F.BC_PERIODIC,
+                                  boundary_z=F.BC_PERIODIC, viscosity_active=1, mu=0.2)
+    o = Oracle(p)
+    y = (np.arange(p.Nty) - p.jbeg + 0.5) * p.dy
+    u = np.sin(2 * np.pi * y)
+    Q = o.zeros()
+    Q[..., 0] = 1.0
+    Q[..., 4] = 1.0
+    Q[..., 1] = u[None, :, None]
+    o.fill_boundaries(Q)
+    Unew = np.zeros_like(Q)
+    dt = 1e-3
+    o.viscosity(Q, Unew, dt)
+    d = dom(p, Unew)[1, :, 1]
+    uj = u[p.jbeg:p.jend]
+    up, um = u[p.jbeg + 1:p.jend + 1], u[p.jbeg - 1:p.jend - 1]
+    assert np.allclose(d[:, 1], dt * 0.2 * (up - 2 * uj + um) / p.dy, rtol=1e-12, atol=1e-16)
+    assert np.allclose(d[:, 2], 0.0, atol=1e-16)
+    assert np.allclose(d[:, 3], dt * 0.2 * (-2.0 / 3.0) * (0.25 / p.dx) * (um - up), rtol=1e-12, atol=1e-16)
+    Q[..., 1] = 0.7
+    Q[..., 2] = -0.3
+    Q[..., 3] = 0.2
+    Unew = np.zeros_like(Q)
+    o.viscosity(Q, Unew, dt)
+    assert np.all(dom(p, Unew) == 0.0)
