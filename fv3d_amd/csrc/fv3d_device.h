@@ -89,38 +89,44 @@ __device__ __forceinline__ real minmod(real a, real b) {
 // RiemannSolvers.h:38-112.  Only the selected star side is evaluated; each
 // value that is used is produced by the reference's own expression.
 template <typename real>
-__device__ __forceinline__ void hllc(const Prim<real>& L, const Prim<real>& R, real gamma,
+__device__ __forceinline__ void hllc(const Prim<real> L, const Prim<real> R, real gamma,
                                      real entho, real F[5], real& pout) {
-  const real ekL = (real)0.5 * L.r * (L.u * L.u + L.v * L.v);  // drops the 2nd transverse velocity
-  const real EL = ekL + L.p * entho;
-  const real ekR = (real)0.5 * R.r * (R.u * R.u + R.v * R.v);
-  const real ER = ekR + R.p * entho;
-  const real cL = sound_speed(L.r, L.p, gamma);
-  const real cR = sound_speed(R.r, R.p, gamma);
+  // states by value and selects on scalars: a select between the addresses
+  // of two stack objects would force them into scratch memory
+  const real rL = L.r, uL = L.u, vL = L.v, wL = L.w, pL = L.p;
+  const real rR = R.r, uR = R.u, vR = R.v, wR = R.w, pR = R.p;
+  const real ekL = (real)0.5 * rL * (uL * uL + vL * vL);  // drops the 2nd transverse velocity
+  const real EL = ekL + pL * entho;
+  const real ekR = (real)0.5 * rR * (uR * uR + vR * vR);
+  const real ER = ekR + pR * entho;
+  const real cL = sound_speed(rL, pL, gamma);
+  const real cR = sound_speed(rR, pR, gamma);
   const real cmax = fmax(cL, cR);
-  const real SL = fmin(L.u, R.u) - cmax;
-  const real SR = fmax(L.u, R.u) + cmax;
-  const real rcL = L.r * (L.u - SL);
-  const real rcR = R.r * (SR - R.u);
-  const real uS = (rcR * R.u + rcL * L.u + (L.p - R.p)) / (rcR + rcL);
-  const real pS = (rcR * L.p + rcL * R.p + rcL * rcR * (L.u - R.u)) / (rcR + rcL);
+  const real SL = fmin(uL, uR) - cmax;
+  const real SR = fmax(uL, uR) + cmax;
+  const real rcL = rL * (uL - SL);
+  const real rcR = rR * (SR - uR);
+  const real uS = (rcR * uR + rcL * uL + (pL - pR)) / (rcR + rcL);
+  const real pS = (rcR * pL + rcL * pR + rcL * rcR * (uL - uR)) / (rcR + rcL);
   // star side K: left when uS > 0, else right
   const bool kl = uS > (real)0;
-  const real rK = kl ? L.r : R.r, uK = kl ? L.u : R.u, pK = kl ? L.p : R.p;
-  const real EK = kl ? EL : ER, SK = kl ? SL : SR;
-  const real vK = kl ? L.v : R.v, wK = kl ? L.w : R.w;
+  const real rK = kl ? rL : rR;
+  const real uK = kl ? uL : uR;
+  const real pK = kl ? pL : pR;
+  const real EK = kl ? EL : ER;
+  const real SK = kl ? SL : SR;
   const real rS = rK * (SK - uK) / (SK - uS);
   const real ES = ((SK - uK) * EK - pK * uK + pS * uS) / (SK - uS);
-  const bool star = !(SL > (real)0) && (kl || SR > (real)0);
   const bool left = SL > (real)0;
-  real sr, su, sv, sw, sp, E;
-  if (star) {
-    sr = rS; su = uS; sv = vK; sw = wK; sp = pS; E = ES;
-  } else if (left) {
-    sr = L.r; su = L.u; sv = L.v; sw = L.w; sp = L.p; E = EL;
-  } else {
-    sr = R.r; su = R.u; sv = R.v; sw = R.w; sp = R.p; E = ER;
-  }
+  const bool star = !left && (kl || SR > (real)0);
+  // non-star side: left if SL > 0 else right
+  const real sr = star ? rS : (left ? rL : rR);
+  const real su = star ? uS : (left ? uL : uR);
+  const real sp = star ? pS : (left ? pL : pR);
+  const real E = star ? ES : (left ? EL : ER);
+  const bool vw_left = star ? kl : left;
+  const real sv = vw_left ? vL : vR;
+  const real sw = vw_left ? wL : wR;
   pout = sp;
   F[0] = sr * su;
   F[1] = sr * su * su + sp;
@@ -132,7 +138,7 @@ __device__ __forceinline__ void hllc(const Prim<real>& L, const Prim<real>& R, r
 // RiemannSolvers.h:5-36 with computeFlux (States.h:113-127) and primToCons
 // (States.h:20-31).  p/(gamma0-1) appears in both and is computed once.
 template <typename real>
-__device__ __forceinline__ void hll(const Prim<real>& L, const Prim<real>& R, real gamma, real gm1,
+__device__ __forceinline__ void hll(const Prim<real> L, const Prim<real> R, real gamma, real gm1,
                                     real F[5], real& pout) {
   const real aL = sound_speed(L.r, L.p, gamma);
   const real aR = sound_speed(R.r, R.p, gamma);
@@ -169,7 +175,7 @@ __device__ __forceinline__ void hll(const Prim<real>& L, const Prim<real>& R, re
 }
 
 template <typename real, int RS>
-__device__ __forceinline__ void riemann(const Prim<real>& L, const Prim<real>& R, real gamma,
+__device__ __forceinline__ void riemann(const Prim<real> L, const Prim<real> R, real gamma,
                                         real gm1, real entho, real F[5], real& pout) {
   if (RS == FV3D_HLL) hll(L, R, gamma, gm1, F, pout);
   else hllc(L, R, gamma, entho, F, pout);

@@ -15,9 +15,11 @@
 // Work layout.  A workgroup owns a tile of 64 (x) by TY (y) columns and
 // marches through a chunk of z planes.  Its window is (TY+4) rows by 68
 // columns (2 halo cells each side).  Waves 0..TY+3 own one window row each
-// (lane = x, coalesced loads); wave TY+4 owns the 4 halo columns of every
-// row.  Every owner keeps its cell's primitives for planes k, k+1, k+2 in
-// registers, so z neighbours, z slopes and z faces never touch LDS.  Per
+// (lane = x, coalesced loads); the four halo-row waves also own the 4 halo
+// columns of every row.  Every row owner keeps its cell's primitives for
+// planes k, k+1, k+2 in registers, so z neighbours, z slopes and z faces
+// never touch LDS.  TY+4 = 12 waves leave 168 VGPRs per lane (3 waves per
+// SIMD), which the FP64 solver needs without spilling.  Per
 // plane the owners publish plane k to LDS, waves compute each x and y face
 // once (the reference solves each face twice, Update.h:125-126, with
 // identical inputs), and the TY interior waves update their cells.
@@ -105,7 +107,7 @@ __device__ __forceinline__ void face_flux4(const Prim<real>& a, const Prim<real>
 }
 
 template <typename real, int TY, int RECON, int RS, bool TC, bool VISC>
-__global__ void __launch_bounds__(64 * (TY + 5))
+__global__ void __launch_bounds__(64 * (TY + 4))
 stage_kernel(const real* __restrict__ Uin, const real* Urk, real* Uout,
              const unsigned long long* __restrict__ inv_cur, unsigned long long* inv_next,
              unsigned long long* inv_zero, int* health, DevParams P) {
@@ -116,7 +118,7 @@ stage_kernel(const real* __restrict__ Uin, const real* Urk, real* Uout,
   __shared__ real svp[VISC ? 3 : 1][VISC ? NR : 1][VISC ? NC : 1];
   __shared__ real sfx[5][TY][WX + 1];
   __shared__ real sfy[5][TY + 1][WX];
-  __shared__ double sred[TY + 5];
+  __shared__ double sred[NR];
 
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -134,18 +136,10 @@ stage_kernel(const real* __restrict__ Uin, const real* Urk, real* Uout,
   const int i0 = tx * WX, j0 = ty * TY, k0 = tz * P.nzc;
   const int k1 = min(k0 + P.nzc, P.Nz);
 
-  // the window cell this thread owns
-  int c, r;
-  bool own = true;
-  if (wave < NR) {
-    r = wave;
-    c = lane + 2;
-  } else {
-    r = lane >> 2;
-    const int qq = lane & 3;
-    c = qq < 2 ? qq : WX + qq;
-    if (lane >= 4 * NR) { own = false; r = 0; c = 0; }
-  }
+  // every wave owns one window row (lane = column 2..65); the four halo-row
+  // waves also own, in lanes 0..NR-1, one of the four halo columns 0, 1,
+  // 66, 67 of every row
+  const int r = wave, c = lane + 2;
   const int gi = i0 - 2 + c, gj = j0 - 2 + r;
   bool fx, fy;
   const int si = map_coord(gi, P.Nx, P.bcx, fx);
@@ -155,6 +149,14 @@ stage_kernel(const real* __restrict__ Uin, const real* Urk, real* Uout,
   const bool interior = wave >= 2 && wave < TY + 2;
   const bool yworker = wave >= 2 && wave < TY + 3;
   const bool store = interior && gi < P.Nx && gj < P.Ny;
+  const int hidx = wave < 2 ? wave : wave - (NR - 4);
+  const bool hown = (wave < 2 || wave >= NR - 2) && lane < NR;
+  const int hr = lane < NR ? lane : 0;
+  const int hc = hidx < 2 ? hidx : WX + hidx;
+  bool hfx, hfy;
+  const int hsi = map_coord(i0 - 2 + hc, P.Nx, P.bcx, hfx);
+  const int hsj = map_coord(j0 - 2 + hr, P.Ny, P.bcy, hfy);
+  const long long hxy = (long long)hsi + (long long)hsj * P.sy;
 
   const real gm1 = (real)P.gm1;
   const double dtd = dt_from_inv(u64_as_double(*inv_cur), P);
@@ -166,20 +168,41 @@ stage_kernel(const real* __restrict__ Uin, const real* Urk, real* Uout,
     const int sk = map_coord(kk, P.Nz, P.bcz, fz);
     return load_prim(Uin, sxy + (long long)sk * P.sz, P.sf, gm1, fx, fy, fz);
   };
+  auto load_halo = [&](int kk) -> Prim<real> {
+    bool fz;
+    const int sk = map_coord(kk, P.Nz, P.bcz, fz);
+    return load_prim(Uin, hxy + (long long)sk * P.sz, P.sf, gm1, hfx, hfy, fz);
+  };
+  // raw conserved values of the halo cell at plane kk, converted later
+  auto fetch_halo = [&](int kk, real h[5]) {
+    bool fz;
+    const int sk = map_coord(kk, P.Nz, P.bcz, fz);
+    const long long base = hxy + (long long)sk * P.sz;
+#pragma unroll
+    for (int n = 0; n < 5; ++n) h[n] = Uin[base + n * P.sf];
+  };
   auto lds_prim = [&](int rr, int cc) -> Prim<real> {
     return Prim<real>{sq[0][rr][cc], sq[1][rr][cc], sq[2][rr][cc], sq[3][rr][cc], sq[4][rr][cc]};
   };
 
+  // Registers carried from plane to plane.  Interior waves: cz = z slope at
+  // k, fz/pz = z face k-1/2 and its pressure.  Halo waves: cz = conserved
+  // state of their halo cell at the next plane (prefetched one plane ahead).
+  real cz[5] = {0, 0, 0, 0, 0}, fz[5] = {0, 0, 0, 0, 0}, pz = 0;
+
   // ---- warm-up: planes k0-2 .. k0+1, z slope at k0 and the z face k0-1/2
-  Prim<real> Pm1 = load_plane(k0 - 1);
+  const Prim<real> Pm1 = load_plane(k0 - 1);
   Prim<real> P0 = load_plane(k0);
   Prim<real> P1 = load_plane(k0 + 1);
-  Prim<real> sK{};
-  real Fzl[5] = {0, 0, 0, 0, 0}, poutl = 0;
   if (interior) {
     const Prim<real> Pm2 = load_plane(k0 - 2);
-    if (RECON == FV3D_PLM) sK = slope3<real, true>(Pm1, P0, P1);
-    face_flux4<real, FV3D_IZ, RECON, RS>(Pm2, Pm1, P0, P1, P, Fzl, poutl);
+    if (RECON == FV3D_PLM) {
+      const Prim<real> s = slope3<real, true>(Pm1, P0, P1);
+      cz[0] = s.r; cz[1] = s.u; cz[2] = s.v; cz[3] = s.w; cz[4] = s.p;
+    }
+    face_flux4<real, FV3D_IZ, RECON, RS>(Pm2, Pm1, P0, P1, P, fz, pz);
+  } else if (hown) {
+    fetch_halo(k0, cz);
   }
   real Tm = 0, T0 = 0, T1 = 0;
   if (TC) { Tm = Pm1.p / Pm1.r; T0 = P0.p / P0.r; T1 = P1.p / P1.r; }
@@ -197,16 +220,34 @@ stage_kernel(const real* __restrict__ Uin, const real* Urk, real* Uout,
     const Prim<real> P2 = load_plane(k + 2);
     real T2 = 0;
     if (TC) T2 = P2.p / P2.r;
+    Prim<real> H{}, Hm{}, Hp{};
+    if (hown) {
+      bool hfz;
+      map_coord(k, P.Nz, P.bcz, hfz);
+      H = cons_to_prim(cz[0], cz[1], cz[2], cz[3], cz[4], gm1);
+      if (hfx) H.u = -H.u;
+      if (hfy) H.v = -H.v;
+      if (hfz) H.w = -H.w;
+      if (VISC) { Hm = load_halo(k - 1); Hp = load_halo(k + 1); }
+    }
 
     __syncthreads();  // readers of the previous plane are done
-    if (own) {
-      sq[0][r][c] = P0.r; sq[1][r][c] = P0.u; sq[2][r][c] = P0.v; sq[3][r][c] = P0.w;
-      sq[4][r][c] = P0.p;
-      if (TC) sT[r][c] = T0;
+    sq[0][r][c] = P0.r; sq[1][r][c] = P0.u; sq[2][r][c] = P0.v; sq[3][r][c] = P0.w;
+    sq[4][r][c] = P0.p;
+    if (TC) sT[r][c] = T0;
+    if (VISC) {
+      svm[0][r][c] = um; svm[1][r][c] = vm; svm[2][r][c] = wm;
+      svp[0][r][c] = P1.u; svp[1][r][c] = P1.v; svp[2][r][c] = P1.w;
+    }
+    if (hown) {
+      sq[0][hr][hc] = H.r; sq[1][hr][hc] = H.u; sq[2][hr][hc] = H.v; sq[3][hr][hc] = H.w;
+      sq[4][hr][hc] = H.p;
+      if (TC) sT[hr][hc] = H.p / H.r;
       if (VISC) {
-        svm[0][r][c] = um; svm[1][r][c] = vm; svm[2][r][c] = wm;
-        svp[0][r][c] = P1.u; svp[1][r][c] = P1.v; svp[2][r][c] = P1.w;
+        svm[0][hr][hc] = Hm.u; svm[1][hr][hc] = Hm.v; svm[2][hr][hc] = Hm.w;
+        svp[0][hr][hc] = Hp.u; svp[1][hr][hc] = Hp.v; svp[2][hr][hc] = Hp.w;
       }
+      if (k + 1 < k1) fetch_halo(k + 1, cz);  // consumed next plane
     }
     __syncthreads();
 
@@ -217,7 +258,7 @@ stage_kernel(const real* __restrict__ Uin, const real* Urk, real* Uout,
                                            lds_prim(r, c + 1), P, F, po);
 #pragma unroll
       for (int n = 0; n < 5; ++n) sfx[n][r - 2][lane] = F[n];
-    } else if (wave == NR && lane < TY) {
+    } else if (wave == NR - 1 && lane < TY) {
       // the right face of column 63 for row lane+2
       real F[5], po;
       face_flux4<real, FV3D_IX, RECON, RS>(lds_prim(lane + 2, WX), lds_prim(lane + 2, WX + 1),
@@ -229,18 +270,18 @@ stage_kernel(const real* __restrict__ Uin, const real* Urk, real* Uout,
     // ---- y faces: face between window rows r-1 and r
     if (yworker) {
       real F[5], po;
-      face_flux4<real, FV3D_IY, RECON, RS>(lds_prim(r - 2, c), lds_prim(r - 1, c),
-                                           (r < TY + 2) ? P0 : lds_prim(r, c),
+      face_flux4<real, FV3D_IY, RECON, RS>(lds_prim(r - 2, c), lds_prim(r - 1, c), P0,
                                            lds_prim(r + 1, c), P, F, po);
 #pragma unroll
       for (int n = 0; n < 5; ++n) sfy[n][r - 2][lane] = F[n];
     }
     // ---- z face k+1/2 from registers
-    real Fzh[5], pouth = 0;
+    real Fzh[5] = {0, 0, 0, 0, 0}, pouth = 0;
     Prim<real> sK1{};
     if (interior) {
       Prim<real> qL, qR;
       if (RECON == FV3D_PLM) {
+        const Prim<real> sK{cz[0], cz[1], cz[2], cz[3], cz[4]};
         sK1 = slope3<real, true>(P0, P1, P2);
         qL = plus_face(P0, sK);
         qR = minus_face(P1, sK1);
@@ -276,17 +317,17 @@ stage_kernel(const real* __restrict__ Uin, const real* Urk, real* Uout,
       }
       real zl[5], zh[5];
 #pragma unroll
-      for (int n = 0; n < 5; ++n) { zl[n] = Fzl[n]; zh[n] = Fzh[n]; }
+      for (int n = 0; n < 5; ++n) { zl[n] = fz[n]; zh[n] = Fzh[n]; }
       if (P.wb_flux_zbc && (k == 0 || k == P.Nz - 1)) {  // Update.h:132-137
         if (k == 0) {
           zl[0] = 0; zl[1] = 0; zl[2] = 0; zl[4] = 0;
           zl[3] = pouth - P0.r * (real)P.g * (real)P.dz;
         } else {
           zh[0] = 0; zh[1] = 0; zh[2] = 0; zh[4] = 0;
-          zh[3] = poutl + P0.r * (real)P.g * (real)P.dz;
+          zh[3] = pz + P0.r * (real)P.g * (real)P.dz;
         }
       } else if (P.c91_flux_zbc && (k == 0 || k == P.Nz - 1)) {  // Update.h:138-143
-        if (k == 0) { zl[0] = 0; zl[1] = 0; zl[2] = 0; zl[3] = poutl; zl[4] = 0; }
+        if (k == 0) { zl[0] = 0; zl[1] = 0; zl[2] = 0; zl[3] = pz; zl[4] = 0; }
         else { zh[0] = 0; zh[1] = 0; zh[2] = 0; zh[3] = pouth; zh[4] = 0; }
       }
 #pragma unroll
@@ -439,10 +480,12 @@ stage_kernel(const real* __restrict__ Uin, const real* Urk, real* Uout,
     if (TC) { Tm = T0; T0 = T1; T1 = T2; }
     P0 = P1;
     P1 = P2;
-    if (RECON == FV3D_PLM) sK = sK1;
+    if (interior) {
+      if (RECON == FV3D_PLM) { cz[0] = sK1.r; cz[1] = sK1.u; cz[2] = sK1.v; cz[3] = sK1.w; cz[4] = sK1.p; }
 #pragma unroll
-    for (int n = 0; n < 5; ++n) Fzl[n] = Fzh[n];
-    poutl = pouth;
+      for (int n = 0; n < 5; ++n) fz[n] = Fzh[n];
+      pz = pouth;
+    }
   }
 
   if (inv_next) {
@@ -455,7 +498,7 @@ stage_kernel(const real* __restrict__ Uin, const real* Urk, real* Uout,
     __syncthreads();
     if (threadIdx.x == 0) {
       double m = 0.0;
-      for (int w = 0; w < TY + 5; ++w) m = (sred[w] > m) ? sred[w] : m;
+      for (int w = 0; w < NR; ++w) m = (sred[w] > m) ? sred[w] : m;
       if (m > 0.0) atomicMax(inv_next, double_as_u64(m));
     }
     if (bad) atomicOr(health, 1);
@@ -470,7 +513,7 @@ static hipError_t launch_t(const StageLaunch<real>& L, hipStream_t s) {
   DevParams P = L.P;
   const int blocks = P.tiles_x * P.tiles_y * P.tiles_z;
   hipLaunchKernelGGL((stage_kernel<real, TY, RECON, RS, TC, VISC>), dim3(blocks),
-                     dim3(64 * (TY + 5)), 0, s, L.Uin, L.Urk, L.Uout, L.inv_cur, L.inv_next,
+                     dim3(64 * (TY + 4)), 0, s, L.Uin, L.Urk, L.Uout, L.inv_cur, L.inv_next,
                      L.inv_zero, L.health, P);
   return hipGetLastError();
 }
