@@ -112,8 +112,13 @@ stage_kernel(const real* __restrict__ Uin, const real* Urk, real* Uout,
              const unsigned long long* __restrict__ inv_cur, unsigned long long* inv_next,
              unsigned long long* inv_zero, int* health, DevParams P) {
   constexpr int NR = TY + 4;
-  __shared__ real sq[5][NR][NC];
-  __shared__ real sT[TC ? NR : 1][TC ? NC : 1];
+  // The primitive plane (and the temperature plane) are double-buffered by
+  // plane parity unless viscosity is on, so the next plane can be published
+  // while slower waves still read this one; viscosity keeps one buffer (its
+  // extra planes leave no LDS for a second) and a barrier instead.
+  constexpr int NBUF = VISC ? 1 : 2;
+  __shared__ real sqb[NBUF][5][NR][NC];
+  __shared__ real sTb[NBUF][TC ? NR : 1][TC ? NC : 1];
   __shared__ real svm[VISC ? 3 : 1][VISC ? NR : 1][VISC ? NC : 1];
   __shared__ real svp[VISC ? 3 : 1][VISC ? NR : 1][VISC ? NC : 1];
   __shared__ real sfx[5][TY][WX + 1];
@@ -181,9 +186,6 @@ stage_kernel(const real* __restrict__ Uin, const real* Urk, real* Uout,
 #pragma unroll
     for (int n = 0; n < 5; ++n) h[n] = Uin[base + n * P.sf];
   };
-  auto lds_prim = [&](int rr, int cc) -> Prim<real> {
-    return Prim<real>{sq[0][rr][cc], sq[1][rr][cc], sq[2][rr][cc], sq[3][rr][cc], sq[4][rr][cc]};
-  };
 
   // Registers carried from plane to plane.  Interior waves: cz = z slope at
   // k, fz/pz = z face k-1/2 and its pressure.  Halo waves: cz = conserved
@@ -217,6 +219,11 @@ stage_kernel(const real* __restrict__ Uin, const real* Urk, real* Uout,
   int bad = 0;
 
   for (int k = k0; k < k1; ++k) {
+    real (*sq)[NR][NC] = sqb[NBUF == 2 ? (k & 1) : 0];
+    real (*sT)[TC ? NC : 1] = sTb[NBUF == 2 ? (k & 1) : 0];
+    auto lds_prim = [&](int rr, int cc) -> Prim<real> {
+      return Prim<real>{sq[0][rr][cc], sq[1][rr][cc], sq[2][rr][cc], sq[3][rr][cc], sq[4][rr][cc]};
+    };
     const Prim<real> P2 = load_plane(k + 2);
     real T2 = 0;
     if (TC) T2 = P2.p / P2.r;
@@ -231,7 +238,7 @@ stage_kernel(const real* __restrict__ Uin, const real* Urk, real* Uout,
       if (VISC) { Hm = load_halo(k - 1); Hp = load_halo(k + 1); }
     }
 
-    __syncthreads();  // readers of the previous plane are done
+    if (NBUF == 1) __syncthreads();  // readers of the previous plane are done
     sq[0][r][c] = P0.r; sq[1][r][c] = P0.u; sq[2][r][c] = P0.v; sq[3][r][c] = P0.w;
     sq[4][r][c] = P0.p;
     if (TC) sT[r][c] = T0;
@@ -250,6 +257,19 @@ stage_kernel(const real* __restrict__ Uin, const real* Urk, real* Uout,
       if (k + 1 < k1) fetch_halo(k + 1, cz);  // consumed next plane
     }
     __syncthreads();
+
+    // the update's base state (and U^n of the RK2 combine) are fetched here,
+    // so their latency hides behind the face solves
+    const long long cell = (long long)gi + (long long)gj * P.sy + (long long)k * P.sz;
+    real ub[5], ur[5];
+    if (store) {
+#pragma unroll
+      for (int n = 0; n < 5; ++n) ub[n] = Uin[cell + n * P.sf];
+      if (Urk) {
+#pragma unroll
+        for (int n = 0; n < 5; ++n) ur[n] = Urk[cell + n * P.sf];
+      }
+    }
 
     // ---- x faces: face between window columns c-1 and c for every interior row
     if (interior) {
@@ -301,10 +321,9 @@ stage_kernel(const real* __restrict__ Uin, const real* Urk, real* Uout,
 
     // ---- update (Update.h:145-160, then ThermalConduction.h, Viscosity.h)
     if (store) {
-      const long long cell = (long long)gi + (long long)gj * P.sy + (long long)k * P.sz;
       real u[5];
 #pragma unroll
-      for (int n = 0; n < 5; ++n) u[n] = Uin[cell + n * P.sf];
+      for (int n = 0; n < 5; ++n) u[n] = ub[n];
 #pragma unroll
       for (int n = 0; n < 5; ++n) {
         const real t = (sfx[n][r - 2][lane] - sfx[n][r - 2][lane + 1]) * dt;
@@ -459,7 +478,7 @@ stage_kernel(const real* __restrict__ Uin, const real* Urk, real* Uout,
 
       if (Urk) {  // Update.h:203
 #pragma unroll
-        for (int n = 0; n < 5; ++n) u[n] = (real)0.5 * (Urk[cell + n * P.sf] + u[n]);
+        for (int n = 0; n < 5; ++n) u[n] = (real)0.5 * (ur[n] + u[n]);
       }
 #pragma unroll
       for (int n = 0; n < 5; ++n) Uout[cell + n * P.sf] = u[n];
