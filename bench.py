@@ -105,6 +105,47 @@ def stream_copy(nbytes=1 << 30, reps=20):
     return F.stream_copy_gbs(nbytes, reps)
 
 
+def measure(F, p, Q0, precision, steps, warmup):
+    """Timed steps of one precision: HIP events on the launch stream around
+    every stage launch.  The reference scheme itself develops negative
+    pressure on the 256^3 blast after ~90 RK2 steps (the oracle reproduces it
+    bit for bit), so steps are timed in windows of at most CHUNK steps, each
+    started (untimed) from the initial condition plus the warm-up steps."""
+    CHUNK = 60
+    sim = F.Simulation(p, precision)
+    total_ms, stage_ms, wall = 0.0, [], 0.0
+    left = steps
+    while left > 0:
+        n = min(CHUNK, left)
+        sim.set_prim(Q0, with_ghosts=True)
+        if warmup > 0:
+            sim.step(warmup)
+        sim.synchronize()
+        dt0, _ = sim.compute_dt()
+        if not np.isfinite(dt0) or dt0 <= 0:
+            raise RuntimeError("non-finite dt after warm-up")
+        t0 = time.perf_counter()
+        ms, st = sim.time_steps(n, per_stage=True)
+        sim.synchronize()
+        wall += time.perf_counter() - t0
+        sim.compute_dt()  # raises if the window produced non-finite values
+        total_ms += ms
+        stage_ms.extend(st.tolist())
+        left -= n
+    sim.close()
+    return total_ms, stage_ms, wall
+
+
+def form_line(F, p, Q0, precision, steps, warmup, alg_bytes):
+    cells = p.Nx * p.Ny * p.Nz
+    total_ms, stage_ms, _ = measure(F, p, Q0, precision, steps, warmup)
+    value = cells * steps / (total_ms * 1e-3) / 1e9
+    achieved = alg_bytes * cells / 2.0 / (float(np.mean(stage_ms)) * 1e-3) / 1e9
+    return {"value": round(value, 4), "unit": "Gcell-updates/s", "ms_per_step": round(total_ms / steps, 4),
+            "alg_bytes_per_cell_update": alg_bytes, "roofline_achieved_GBs": round(achieved, 1),
+            "roofline_frac": round(achieved / HBM_PEAK_GBS, 4)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,6 +155,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU-baseline timing (0: skip)")
     ap.add_argument("--cpu-n", type=int, default=256, help="cells per axis of the CPU-baseline sample")
     ap.add_argument("--stream", type=int, default=1, help="also measure the device copy bandwidth")
+    ap.add_argument("--forms", type=int, default=1, help="also time the FP32 forms")
     ap.add_argument("--copy-only", action="store_true",
                     help="run only the 1 GiB copy (calibration run for the rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -131,40 +173,8 @@ def main():
     import fv3d_amd as F
     p = blast_params(args.n)
     cells = p.Nx * p.Ny * p.Nz
-    sim = F.Simulation(p, F.FP64)
     Q0 = blast_ic(p)
-
-    def reseed():
-        sim.set_prim(Q0, with_ghosts=True)
-        if args.warmup > 0:
-            sim.step(args.warmup)
-        sim.synchronize()
-        dt0, _ = sim.compute_dt()
-        if not np.isfinite(dt0) or dt0 <= 0:
-            raise RuntimeError("non-finite dt after warm-up")
-
-    # The reference scheme itself develops negative pressure on this blast
-    # after ~90 RK2 steps at 256^3 (the oracle reproduces it bit for bit), so
-    # the timed steps are taken in windows of at most CHUNK steps, each
-    # started (untimed) from the initial condition plus the warm-up steps.
-    CHUNK = 60
-    total_ms = 0.0
-    stage_ms = []
-    wall = 0.0
-    left = args.steps
-    while left > 0:
-        n = min(CHUNK, left)
-        reseed()
-        t0 = time.perf_counter()
-        ms, st = sim.time_steps(n, per_stage=True)  # HIP events on the launch stream
-        sim.synchronize()
-        wall += time.perf_counter() - t0
-        dt1, _ = sim.compute_dt()  # raises if the window produced non-finite values
-        total_ms += ms
-        stage_ms.extend(st.tolist())
-        left -= n
-    sim.close()
-
+    total_ms, stage_ms, wall = measure(F, p, Q0, F.FP64, args.steps, args.warmup)
     ms_per_step = total_ms / args.steps
     value = cells * args.steps / (total_ms * 1e-3) / 1e9
     mean_launch_ms = float(np.mean(stage_ms))
@@ -205,6 +215,12 @@ def main():
     }
     if args.stream:
         out["roofline"]["stream_copy_GBs"] = round(stream_copy(), 1)
+    if args.forms:
+        # the FP32 forms on the same workload (error against FP64 in tests/)
+        out["forms"] = {
+            "fp32_kahan_rk2": form_line(F, p, Q0, F.FP32, args.steps, args.warmup, 140.0),
+            "fp32_plain_rk2": form_line(F, p, Q0, F.FP32_PLAIN, args.steps, args.warmup, 100.0),
+        }
     if args.cpu_budget > 0:
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_budget)
     else:

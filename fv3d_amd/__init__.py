@@ -32,7 +32,7 @@ TCM_CONSTANT, TCM_B02 = 0, 1
 BCTC_NONE, BCTC_FIXED_TEMPERATURE, BCTC_FIXED_GRADIENT = 0, 1, 2
 VSC_CONSTANT = 0
 
-FP64, FP32 = 0, 1
+FP64, FP32, FP32_PLAIN = 0, 1, 2
 
 ERRORS = {1: "EINVAL", 2: "EHIP", 3: "ENOMEM", 4: "ENONFINITE"}
 

@@ -29,7 +29,9 @@ HIP_FLAGS = [
     "-Wno-unused-value", "-Wno-unused-result", f"-I{INCLUDE}", f"-I{CSRC}",
 ]
 
-SOURCES = ["fv3d_stage.hip", "fv3d_aux.hip", "fv3d_api.hip"]
+SOURCES = ["fv3d_stage.hip", "fv3d_stage_f32.hip", "fv3d_aux.hip", "fv3d_api.hip"]
+# per-source extra flags
+EXTRA = {"fv3d_stage_f32.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt"]}
 
 
 def _run(cmd):
@@ -47,7 +49,7 @@ def build_library(verbose: bool = False) -> str:
     for src in SOURCES:
         obj = os.path.join(BUILD, src.replace(".hip", ".o"))
         objs.append(obj)
-        cmds.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+        cmds.append([HIPCC, *HIP_FLAGS, *EXTRA.get(src, []), "-c", os.path.join(CSRC, src), "-o", obj])
     with ThreadPoolExecutor(max_workers=min(len(cmds), 8)) as ex:
         list(ex.map(_run, cmds))
     tmp = LIB + ".tmp"

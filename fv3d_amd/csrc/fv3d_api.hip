@@ -6,6 +6,10 @@
 //   RK2:   stage(A -> B)               reads slot s,   clears slot s+2
 //          stage(B, rk=A -> A)         reads slot s,   reduces slot s+1
 //   Euler: stage(A -> B), swap(A, B)   reads slot s,   reduces s+1, clears s+2
+// The FP32 compensated RK2 adds a third state buffer C and a compensation
+// array:
+//          inc1(A -> K in B)           reads slot s,   clears slot s+2
+//          inc2(A, K, comp -> C), swap(A, C)   reduces slot s+1
 // so dt never round-trips through the host and consecutive steps need no
 // synchronisation.  The slot cleared by step s was last read by step s-1 and
 // is next written by step s+1, so clearing it inside step s races with
@@ -29,7 +33,12 @@ struct fv3d_ctx {
   hipStream_t stream = nullptr;
   void* A = nullptr;
   void* B = nullptr;
+  void* C = nullptr;    // FP32 compensated RK2: U^{n+1} target
+  void* Cmp = nullptr;  // FP32 compensated RK2: Kahan compensation
+  size_t elem = sizeof(double);
   size_t state_bytes = 0;
+  bool stage_is_inc = false;  // B holds the stage-1 increment K (FP32 compensated RK2)
+  void* stage_base = nullptr; // U^n of the last step, when B holds K
   unsigned long long* ring = nullptr;  // 3 slots
   int* health = nullptr;
   double* qbuf = nullptr;  // ghosted SoA primitive staging, allocated on first use
@@ -54,6 +63,10 @@ static int fail(fv3d_ctx* c, int code, const std::string& msg) {
     if (e_ != hipSuccess)                                                              \
       return fail((ctx), FV3D_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
+
+static bool kahan(const fv3d_ctx* c) {
+  return c->precision == FV3D_FP32 && c->p.time_stepping == FV3D_TS_RK2;
+}
 
 static bool is_pow2(double d) {
   if (!(d > 0.0) || !std::isfinite(d)) return false;
@@ -141,7 +154,8 @@ int fv3d_create(const fv3d_params* p, int precision, int device, fv3d_ctx** out)
   *out = nullptr;
   std::string why;
   if (check_params(*p, why)) return fail(nullptr, FV3D_EINVAL, why);
-  if (precision != FV3D_FP64) return fail(nullptr, FV3D_EINVAL, "precision not supported");
+  if (precision != FV3D_FP64 && precision != FV3D_FP32 && precision != FV3D_FP32_PLAIN)
+    return fail(nullptr, FV3D_EINVAL, "precision not supported");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
     return fail(nullptr, FV3D_EHIP, "no HIP device visible");
@@ -151,8 +165,8 @@ int fv3d_create(const fv3d_params* p, int precision, int device, fv3d_ctx** out)
   c->precision = precision;
   c->device = device;
   c->dp = make_devparams(*p);
-  const size_t elem = sizeof(double);
-  c->state_bytes = (size_t)5 * p->Nx * p->Ny * p->Nz * elem;
+  c->elem = precision == FV3D_FP64 ? sizeof(double) : sizeof(float);
+  c->state_bytes = (size_t)5 * p->Nx * p->Ny * p->Nz * c->elem;
   auto cleanup = [&](int code, const std::string& m) {
     fv3d_destroy(c);
     return fail(nullptr, code, m);
@@ -162,6 +176,9 @@ int fv3d_create(const fv3d_params* p, int precision, int device, fv3d_ctx** out)
     return cleanup(FV3D_EHIP, "hipStreamCreate failed");
   if (hipMalloc(&c->A, c->state_bytes) != hipSuccess ||
       hipMalloc(&c->B, c->state_bytes) != hipSuccess)
+    return cleanup(FV3D_ENOMEM, "state allocation failed");
+  if (kahan(c) && (hipMalloc(&c->C, c->state_bytes) != hipSuccess ||
+                   hipMalloc(&c->Cmp, c->state_bytes) != hipSuccess))
     return cleanup(FV3D_ENOMEM, "state allocation failed");
   if (hipMalloc(&c->ring, 4 * sizeof(unsigned long long)) != hipSuccess ||
       hipMalloc(&c->health, sizeof(int)) != hipSuccess)
@@ -179,6 +196,8 @@ int fv3d_destroy(fv3d_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   if (c->A) hipFree(c->A);
   if (c->B) hipFree(c->B);
+  if (c->C) hipFree(c->C);
+  if (c->Cmp) hipFree(c->Cmp);
   if (c->ring) hipFree(c->ring);
   if (c->health) hipFree(c->health);
   if (c->qbuf) hipFree(c->qbuf);
@@ -241,12 +260,18 @@ static int ensure_qbuf(fv3d_ctx* c) {
 }
 
 // prim of a device state buffer into host memory
-static int state_prim_to_host(fv3d_ctx* c, const void* U, double* Q, int with_ghosts) {
+static int state_prim_to_host(fv3d_ctx* c, const void* U, const void* K, double* Q,
+                              int with_ghosts) {
   int rc = ensure_qbuf(c);
   if (rc) return rc;
   const fv3d_params& p = c->p;
   const size_t nt = (size_t)p.Ntx * p.Nty * p.Ntz;
-  HIPCHK(c, launch_cons_to_prim_ghosted<double>((const double*)U, c->qbuf, c->dp, p.Ng, c->stream));
+  if (c->precision == FV3D_FP64)
+    HIPCHK(c, launch_cons_to_prim_ghosted<double>((const double*)U, (const double*)K, c->qbuf,
+                                                  c->dp, p.Ng, c->stream));
+  else
+    HIPCHK(c, launch_cons_to_prim_ghosted<float>((const float*)U, (const float*)K, c->qbuf, c->dp,
+                                                 p.Ng, c->stream));
   if (with_ghosts) HIPCHK(c, launch_fill_boundaries(c->qbuf, p, c->stream));
   std::vector<double> h(5 * nt);
   HIPCHK(c, hipMemcpyAsync(h.data(), c->qbuf, 5 * nt * sizeof(double), hipMemcpyDeviceToHost,
@@ -270,7 +295,11 @@ static int state_prim_to_host(fv3d_ctx* c, const void* U, double* Q, int with_gh
 static int reset_dt(fv3d_ctx* c) {
   HIPCHK(c, launch_fill_zero_u64(c->ring, 4, c->stream));
   HIPCHK(c, hipMemsetAsync(c->health, 0, sizeof(int), c->stream));
-  HIPCHK(c, launch_dt_reduce<double>((const double*)c->A, c->ring, c->health, c->dp, c->stream));
+  if (c->precision == FV3D_FP64)
+    HIPCHK(c, launch_dt_reduce<double>((const double*)c->A, c->ring, c->health, c->dp, c->stream));
+  else
+    HIPCHK(c, launch_dt_reduce<float>((const float*)c->A, c->ring, c->health, c->dp, c->stream));
+  if (c->Cmp) HIPCHK(c, hipMemsetAsync(c->Cmp, 0, c->state_bytes, c->stream));
   c->step = 0;
   c->have_state = true;
   c->have_stage = false;
@@ -288,7 +317,9 @@ int fv3d_set_prim(fv3d_ctx* c, const double* Q, int with_ghosts) {
   HIPCHK(c, hipMalloc(&d, h.size() * sizeof(double)));
   hipError_t e = hipMemcpyAsync(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
                                 c->stream);
-  if (e == hipSuccess) e = launch_prim_to_cons<double>(d, (double*)c->A, c->dp, c->stream);
+  if (e == hipSuccess)
+    e = c->precision == FV3D_FP64 ? launch_prim_to_cons<double>(d, (double*)c->A, c->dp, c->stream)
+                                  : launch_prim_to_cons<float>(d, (float*)c->A, c->dp, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   hipFree(d);
   if (e != hipSuccess) return fail(c, FV3D_EHIP, std::string("set_prim: ") + hipGetErrorString(e));
@@ -300,7 +331,13 @@ int fv3d_set_cons(fv3d_ctx* c, const double* U, int with_ghosts) {
   HIPCHK(c, hipSetDevice(c->device));
   std::vector<double> h;
   aos_to_soa_domain(c->p, U, with_ghosts, h);
-  HIPCHK(c, hipMemcpyAsync(c->A, h.data(), c->state_bytes, hipMemcpyHostToDevice, c->stream));
+  std::vector<float> hf;
+  const void* src = h.data();
+  if (c->precision != FV3D_FP64) {
+    hf.assign(h.begin(), h.end());
+    src = hf.data();
+  }
+  HIPCHK(c, hipMemcpyAsync(c->A, src, c->state_bytes, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return reset_dt(c);
 }
@@ -309,9 +346,17 @@ int fv3d_get_cons(fv3d_ctx* c, double* U, int with_ghosts) {
   if (!c || !U) return fail(c, FV3D_EINVAL, "null argument");
   if (!c->have_state) return fail(c, FV3D_EINVAL, "no state set");
   HIPCHK(c, hipSetDevice(c->device));
-  std::vector<double> h(c->state_bytes / sizeof(double));
-  HIPCHK(c, hipMemcpyAsync(h.data(), c->A, c->state_bytes, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const size_t n = c->state_bytes / c->elem;
+  std::vector<double> h(n);
+  if (c->precision == FV3D_FP64) {
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->A, c->state_bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  } else {
+    std::vector<float> hf(n);
+    HIPCHK(c, hipMemcpyAsync(hf.data(), c->A, c->state_bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    h.assign(hf.begin(), hf.end());
+  }
   soa_domain_to_aos(c->p, h.data(), with_ghosts, U);
   return FV3D_OK;
 }
@@ -320,14 +365,15 @@ int fv3d_get_prim(fv3d_ctx* c, double* Q, int with_ghosts) {
   if (!c || !Q) return fail(c, FV3D_EINVAL, "null argument");
   if (!c->have_state) return fail(c, FV3D_EINVAL, "no state set");
   HIPCHK(c, hipSetDevice(c->device));
-  return state_prim_to_host(c, c->A, Q, with_ghosts);
+  return state_prim_to_host(c, c->A, nullptr, Q, with_ghosts);
 }
 
 int fv3d_get_stage_prim(fv3d_ctx* c, double* Q, int with_ghosts) {
   if (!c || !Q) return fail(c, FV3D_EINVAL, "null argument");
   if (!c->have_stage) return fail(c, FV3D_EINVAL, "no step taken yet");
   HIPCHK(c, hipSetDevice(c->device));
-  return state_prim_to_host(c, c->B, Q, with_ghosts);
+  if (c->stage_is_inc) return state_prim_to_host(c, c->stage_base, c->B, Q, with_ghosts);
+  return state_prim_to_host(c, c->B, nullptr, Q, with_ghosts);
 }
 
 int fv3d_compute_dt(fv3d_ctx* c, double inv[3], double* dt) {
@@ -353,12 +399,13 @@ int fv3d_compute_dt(fv3d_ctx* c, double inv[3], double* dt) {
 
 }  // extern "C"
 
-static int launch_step(fv3d_ctx* c, hipEvent_t* ev) {
+template <typename real>
+static int launch_step_t(fv3d_ctx* c, hipEvent_t* ev) {
   const long s = c->step;
   unsigned long long* cur = c->ring + (s % 3);
   unsigned long long* nxt = c->ring + ((s + 1) % 3);
   unsigned long long* zer = c->ring + ((s + 2) % 3);
-  StageLaunch<double> L;
+  StageLaunch<real> L;
   L.recon = c->p.reconstruction == FV3D_PLM ? FV3D_PLM : FV3D_PCM;
   L.riemann = c->p.riemann_solver;
   L.tc = c->p.thermal_conductivity_active != 0;
@@ -366,26 +413,45 @@ static int launch_step(fv3d_ctx* c, hipEvent_t* ev) {
   L.health = c->health;
   L.P = c->dp;
   L.inv_cur = cur;
-  double* A = (double*)c->A;
-  double* B = (double*)c->B;
-  if (c->p.time_stepping == FV3D_TS_RK2) {
+  real* A = (real*)c->A;
+  real* B = (real*)c->B;
+  c->stage_is_inc = false;
+  if (kahan(c)) {
+    L.mode = 1;  // STAGE_INC1: K = U* - U^n into B
     L.Uin = A; L.Urk = nullptr; L.Uout = B; L.inv_next = nullptr; L.inv_zero = zer;
     if (ev) HIPCHK(c, hipEventRecord(ev[0], c->stream));
-    HIPCHK(c, launch_stage<double>(L, c->stream));
+    HIPCHK(c, launch_stage<real>(L, c->stream));
+    if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
+    L.mode = 2;  // STAGE_INC2: compensated U^{n+1} into C
+    L.Uin = A; L.Kin = B; L.Cio = (real*)c->Cmp; L.Uout = (real*)c->C;
+    L.inv_next = nxt; L.inv_zero = nullptr;
+    HIPCHK(c, launch_stage<real>(L, c->stream));
+    if (ev) HIPCHK(c, hipEventRecord(ev[2], c->stream));
+    std::swap(c->A, c->C);
+    c->stage_is_inc = true;
+    c->stage_base = c->C;
+  } else if (c->p.time_stepping == FV3D_TS_RK2) {
+    L.Uin = A; L.Urk = nullptr; L.Uout = B; L.inv_next = nullptr; L.inv_zero = zer;
+    if (ev) HIPCHK(c, hipEventRecord(ev[0], c->stream));
+    HIPCHK(c, launch_stage<real>(L, c->stream));
     if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
     L.Uin = B; L.Urk = A; L.Uout = A; L.inv_next = nxt; L.inv_zero = nullptr;
-    HIPCHK(c, launch_stage<double>(L, c->stream));
+    HIPCHK(c, launch_stage<real>(L, c->stream));
     if (ev) HIPCHK(c, hipEventRecord(ev[2], c->stream));
   } else {
     L.Uin = A; L.Urk = nullptr; L.Uout = B; L.inv_next = nxt; L.inv_zero = zer;
     if (ev) HIPCHK(c, hipEventRecord(ev[0], c->stream));
-    HIPCHK(c, launch_stage<double>(L, c->stream));
+    HIPCHK(c, launch_stage<real>(L, c->stream));
     if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
     std::swap(c->A, c->B);
   }
   c->step = s + 1;
   c->have_stage = true;
   return FV3D_OK;
+}
+
+static int launch_step(fv3d_ctx* c, hipEvent_t* ev) {
+  return c->precision == FV3D_FP64 ? launch_step_t<double>(c, ev) : launch_step_t<float>(c, ev);
 }
 
 extern "C" {

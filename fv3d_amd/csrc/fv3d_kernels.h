@@ -21,17 +21,20 @@ struct StageLaunch {
   int* health;
   int recon, riemann;
   bool tc, visc;
+  int mode = 0;              // STAGE_PLAIN / STAGE_INC1 / STAGE_INC2 (FP32 only)
+  const real* Kin = nullptr;  // stage-1 increment (STAGE_INC2)
+  real* Cio = nullptr;        // Kahan compensation, read and written (STAGE_INC2)
   DevParams P;
 };
 
 template <typename real>
 hipError_t launch_stage(const StageLaunch<real>& L, hipStream_t s);
 
-// cons -> prim over the domain, written into a ghosted SoA array
-// [5][Ntz][Nty][Ntx] (ghosts untouched)
+// cons -> prim over the domain of U (or of U + K when K is given), written
+// into a ghosted SoA array [5][Ntz][Nty][Ntx] (ghosts untouched)
 template <typename real>
-hipError_t launch_cons_to_prim_ghosted(const real* U, double* Q, const DevParams& P, int Ng,
-                                       hipStream_t s);
+hipError_t launch_cons_to_prim_ghosted(const real* U, const real* K, double* Q, const DevParams& P,
+                                       int Ng, hipStream_t s);
 // prim (SoA domain [5][Nz][Ny][Nx], double) -> cons state
 template <typename real>
 hipError_t launch_prim_to_cons(const double* Q, real* U, const DevParams& P, hipStream_t s);

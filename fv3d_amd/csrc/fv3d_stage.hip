@@ -1,561 +1,13 @@
-// fv3d_stage.hip -- the fused finite-volume stage kernel for CDNA4 (gfx950).
-//
-// One launch advances every interior cell by one Euler stage
-//     Unew = U + dt*L(prim(U))                          (Update.h:164-178)
-// and, when Urk is given, applies the SSP-RK2 combine 0.5*(U0 + Unew)
-// (Update.h:198-204) before the single store.  When inv_next is given the
-// epilogue also reduces the next step's inv_dt_hyp (ComputeDt.h:28-54) from
-// the stored state, so the step never leaves the device for dt.
-//
-// It replaces, in one pass over HBM, the reference kernels K2-K12:
-// ghost fill (as index maps), cons->prim, minmod slopes, reconstruction,
-// HLL/HLLC, flux divergence with gravity and the density floor, thermal
-// conduction and viscosity.
-//
-// Work layout.  A workgroup owns a tile of 64 (x) by TY (y) columns and
-// marches through a chunk of z planes.  Its window is (TY+4) rows by 68
-// columns (2 halo cells each side).  Waves 0..TY+3 own one window row each
-// (lane = x, coalesced loads); the four halo-row waves also own the 4 halo
-// columns of every row.  Every row owner keeps its cell's primitives for
-// planes k, k+1, k+2 in registers, so z neighbours, z slopes and z faces
-// never touch LDS.  TY+4 = 12 waves leave 168 VGPRs per lane (3 waves per
-// SIMD), which the FP64 solver needs without spilling.  Per
-// plane the owners publish plane k to LDS, waves compute each x and y face
-// once (the reference solves each face twice, Update.h:125-126, with
-// identical inputs), and the TY interior waves update their cells.
-#include "fv3d_device.h"
-#include "fv3d_kernels.h"
+// fv3d_stage.hip -- FP64 instantiations of the fused stage kernel
+// (fv3d_stage_impl.h).  FP64 division and square root are correctly rounded,
+// so results are the reference's bit for bit.
+#include "fv3d_stage_impl.h"
 
 namespace fv3d_hip {
 
-constexpr int WX = 64;      // interior columns per tile (one wave)
-constexpr int NC = WX + 4;  // window columns
-
-template <typename real>
-__device__ __forceinline__ Prim<real> load_prim(const real* __restrict__ U, long long base,
-                                                long long sf, real gm1, bool fx, bool fy,
-                                                bool fz) {
-  const real u0 = U[base], u1 = U[base + sf], u2 = U[base + 2 * sf], u3 = U[base + 3 * sf],
-             u4 = U[base + 4 * sf];
-  Prim<real> q = cons_to_prim(u0, u1, u2, u3, u4, gm1);
-  // reflecting ghosts: BoundaryConditions.h:253-258 negates the normal velocity
-  if (fx) q.u = -q.u;
-  if (fy) q.v = -q.v;
-  if (fz) q.w = -q.w;
-  return q;
+template <>
+hipError_t launch_stage<double>(const StageLaunch<double>& L, hipStream_t s) {
+  return launch_mode<double, STAGE_PLAIN>(L, s);
 }
-
-// slope at b from its neighbours a (minus side) and c (plus side).
-// x and y: minmod(backward, forward); z: minmod(forward, backward)
-// (Update.h:66-84).
-template <typename real, bool ZORD>
-__device__ __forceinline__ Prim<real> slope3(const Prim<real>& a, const Prim<real>& b,
-                                             const Prim<real>& c) {
-  Prim<real> s;
-  if (!ZORD) {
-    s.r = minmod(b.r - a.r, c.r - b.r);
-    s.u = minmod(b.u - a.u, c.u - b.u);
-    s.v = minmod(b.v - a.v, c.v - b.v);
-    s.w = minmod(b.w - a.w, c.w - b.w);
-    s.p = minmod(b.p - a.p, c.p - b.p);
-  } else {
-    s.r = minmod(c.r - b.r, b.r - a.r);
-    s.u = minmod(c.u - b.u, b.u - a.u);
-    s.v = minmod(c.v - b.v, b.v - a.v);
-    s.w = minmod(c.w - b.w, b.w - a.w);
-    s.p = minmod(c.p - b.p, b.p - a.p);
-  }
-  return s;
-}
-
-// PLM face states (Update.h:21): q + (slope*sign)*0.5 with sign = +1 on the
-// plus face and -1 on the minus face; (s*-1)*0.5 == -(s*0.5) exactly.
-template <typename real>
-__device__ __forceinline__ Prim<real> plus_face(const Prim<real>& q, const Prim<real>& s) {
-  return Prim<real>{q.r + s.r * (real)0.5, q.u + s.u * (real)0.5, q.v + s.v * (real)0.5,
-                    q.w + s.w * (real)0.5, q.p + s.p * (real)0.5};
-}
-template <typename real>
-__device__ __forceinline__ Prim<real> minus_face(const Prim<real>& q, const Prim<real>& s) {
-  return Prim<real>{q.r - s.r * (real)0.5, q.u - s.u * (real)0.5, q.v - s.v * (real)0.5,
-                    q.w - s.w * (real)0.5, q.p - s.p * (real)0.5};
-}
-
-// Flux through the face between cells b and c (a before b, d after c) along
-// DIR, in the lab frame.
-template <typename real, int DIR, int RECON, int RS>
-__device__ __forceinline__ void face_flux4(const Prim<real>& a, const Prim<real>& b,
-                                           const Prim<real>& c, const Prim<real>& d,
-                                           const DevParams& P, real F[5], real& pout) {
-  Prim<real> qL, qR;
-  if (RECON == FV3D_PLM) {
-    const Prim<real> sb = slope3<real, DIR == FV3D_IZ>(a, b, c);
-    const Prim<real> sc = slope3<real, DIR == FV3D_IZ>(b, c, d);
-    qL = plus_face(b, sb);
-    qR = minus_face(c, sc);
-  } else {
-    qL = b;
-    qR = c;
-    if (DIR == FV3D_IZ && P.pcm_wb) {  // Update.h:27
-      qL.p = b.p + b.r * (real)P.g * (real)P.dz * (real)0.5;
-      qR.p = c.p - c.r * (real)P.g * (real)P.dz * (real)0.5;
-    }
-  }
-  riemann<real, RS>(rotate<real, DIR>(qL), rotate<real, DIR>(qR), (real)P.gamma, (real)P.gm1,
-                    (real)P.entho, F, pout);
-  unrotate_flux<real, DIR>(F);
-}
-
-template <typename real, int TY, int RECON, int RS, bool TC, bool VISC>
-__global__ void __launch_bounds__(64 * (TY + 4))
-stage_kernel(const real* __restrict__ Uin, const real* Urk, real* Uout,
-             const unsigned long long* __restrict__ inv_cur, unsigned long long* inv_next,
-             unsigned long long* inv_zero, int* health, DevParams P) {
-  constexpr int NR = TY + 4;
-  // The primitive plane (and the temperature plane) are double-buffered by
-  // plane parity unless viscosity is on, so the next plane can be published
-  // while slower waves still read this one; viscosity keeps one buffer (its
-  // extra planes leave no LDS for a second) and a barrier instead.
-  constexpr int NBUF = VISC ? 1 : 2;
-  __shared__ real sqb[NBUF][5][NR][NC];
-  __shared__ real sTb[NBUF][TC ? NR : 1][TC ? NC : 1];
-  __shared__ real svm[VISC ? 3 : 1][VISC ? NR : 1][VISC ? NC : 1];
-  __shared__ real svp[VISC ? 3 : 1][VISC ? NR : 1][VISC ? NC : 1];
-  __shared__ real sfx[5][TY][WX + 1];
-  __shared__ real sfy[5][TY + 1][WX];
-  __shared__ double sred[NR];
-
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-
-  // Blocks are dealt round-robin to the 8 XCDs (block b -> XCD b%8).  Give
-  // each XCD a contiguous run of tiles so neighbouring tiles share halo rows
-  // in that XCD's L2.
-  const int nb = gridDim.x, b = blockIdx.x;
-  const int q8 = nb >> 3, r8 = nb & 7, xcd = b & 7, loc = b >> 3;
-  const int tile = xcd * q8 + (xcd < r8 ? xcd : r8) + loc;
-  const int tx = tile % P.tiles_x;
-  const int t2 = tile / P.tiles_x;
-  const int ty = t2 % P.tiles_y;
-  const int tz = t2 / P.tiles_y;
-  const int i0 = tx * WX, j0 = ty * TY, k0 = tz * P.nzc;
-  const int k1 = min(k0 + P.nzc, P.Nz);
-
-  // every wave owns one window row (lane = column 2..65); the four halo-row
-  // waves also own, in lanes 0..NR-1, one of the four halo columns 0, 1,
-  // 66, 67 of every row
-  const int r = wave, c = lane + 2;
-  const int gi = i0 - 2 + c, gj = j0 - 2 + r;
-  bool fx, fy;
-  const int si = map_coord(gi, P.Nx, P.bcx, fx);
-  const int sj = map_coord(gj, P.Ny, P.bcy, fy);
-  const long long sxy = (long long)si + (long long)sj * P.sy;
-
-  const bool interior = wave >= 2 && wave < TY + 2;
-  const bool yworker = wave >= 2 && wave < TY + 3;
-  const bool store = interior && gi < P.Nx && gj < P.Ny;
-  const int hidx = wave < 2 ? wave : wave - (NR - 4);
-  const bool hown = (wave < 2 || wave >= NR - 2) && lane < NR;
-  const int hr = lane < NR ? lane : 0;
-  const int hc = hidx < 2 ? hidx : WX + hidx;
-  bool hfx, hfy;
-  const int hsi = map_coord(i0 - 2 + hc, P.Nx, P.bcx, hfx);
-  const int hsj = map_coord(j0 - 2 + hr, P.Ny, P.bcy, hfy);
-  const long long hxy = (long long)hsi + (long long)hsj * P.sy;
-
-  const real gm1 = (real)P.gm1;
-  const double dtd = dt_from_inv(u64_as_double(*inv_cur), P);
-  const real dt = (real)dtd;
-  if (inv_zero && b == 0 && threadIdx.x == 0) *inv_zero = 0ull;
-
-  auto load_plane = [&](int kk) -> Prim<real> {
-    bool fz;
-    const int sk = map_coord(kk, P.Nz, P.bcz, fz);
-    return load_prim(Uin, sxy + (long long)sk * P.sz, P.sf, gm1, fx, fy, fz);
-  };
-  auto load_halo = [&](int kk) -> Prim<real> {
-    bool fz;
-    const int sk = map_coord(kk, P.Nz, P.bcz, fz);
-    return load_prim(Uin, hxy + (long long)sk * P.sz, P.sf, gm1, hfx, hfy, fz);
-  };
-  // raw conserved values of the halo cell at plane kk, converted later
-  auto fetch_halo = [&](int kk, real h[5]) {
-    bool fz;
-    const int sk = map_coord(kk, P.Nz, P.bcz, fz);
-    const long long base = hxy + (long long)sk * P.sz;
-#pragma unroll
-    for (int n = 0; n < 5; ++n) h[n] = Uin[base + n * P.sf];
-  };
-
-  // Registers carried from plane to plane.  Interior waves: cz = z slope at
-  // k, fz/pz = z face k-1/2 and its pressure.  Halo waves: cz = conserved
-  // state of their halo cell at the next plane (prefetched one plane ahead).
-  real cz[5] = {0, 0, 0, 0, 0}, fz[5] = {0, 0, 0, 0, 0}, pz = 0;
-
-  // ---- warm-up: planes k0-2 .. k0+1, z slope at k0 and the z face k0-1/2
-  const Prim<real> Pm1 = load_plane(k0 - 1);
-  Prim<real> P0 = load_plane(k0);
-  Prim<real> P1 = load_plane(k0 + 1);
-  if (interior) {
-    const Prim<real> Pm2 = load_plane(k0 - 2);
-    if (RECON == FV3D_PLM) {
-      const Prim<real> s = slope3<real, true>(Pm1, P0, P1);
-      cz[0] = s.r; cz[1] = s.u; cz[2] = s.v; cz[3] = s.w; cz[4] = s.p;
-    }
-    face_flux4<real, FV3D_IZ, RECON, RS>(Pm2, Pm1, P0, P1, P, fz, pz);
-  } else if (hown) {
-    fetch_halo(k0, cz);
-  }
-  real Tm = 0, T0 = 0, T1 = 0;
-  if (TC) { Tm = Pm1.p / Pm1.r; T0 = P0.p / P0.r; T1 = P1.p / P1.r; }
-  real um = Pm1.u, vm = Pm1.v, wm = Pm1.w;
-
-  const real kf = (real)P.kface;
-  const real mu = (real)P.mu;
-  const real oodx = (real)1.0 / (real)P.dx, oody = (real)1.0 / (real)P.dy,
-             oodz = (real)1.0 / (real)P.dz;
-  const real dtdx = dt / (real)P.dx, dtdy = dt / (real)P.dy, dtdz = dt / (real)P.dz;
-  double inv_max = 0.0;
-  int bad = 0;
-
-  for (int k = k0; k < k1; ++k) {
-    real (*sq)[NR][NC] = sqb[NBUF == 2 ? (k & 1) : 0];
-    real (*sT)[TC ? NC : 1] = sTb[NBUF == 2 ? (k & 1) : 0];
-    auto lds_prim = [&](int rr, int cc) -> Prim<real> {
-      return Prim<real>{sq[0][rr][cc], sq[1][rr][cc], sq[2][rr][cc], sq[3][rr][cc], sq[4][rr][cc]};
-    };
-    const Prim<real> P2 = load_plane(k + 2);
-    real T2 = 0;
-    if (TC) T2 = P2.p / P2.r;
-    Prim<real> H{}, Hm{}, Hp{};
-    if (hown) {
-      bool hfz;
-      map_coord(k, P.Nz, P.bcz, hfz);
-      H = cons_to_prim(cz[0], cz[1], cz[2], cz[3], cz[4], gm1);
-      if (hfx) H.u = -H.u;
-      if (hfy) H.v = -H.v;
-      if (hfz) H.w = -H.w;
-      if (VISC) { Hm = load_halo(k - 1); Hp = load_halo(k + 1); }
-    }
-
-    if (NBUF == 1) __syncthreads();  // readers of the previous plane are done
-    sq[0][r][c] = P0.r; sq[1][r][c] = P0.u; sq[2][r][c] = P0.v; sq[3][r][c] = P0.w;
-    sq[4][r][c] = P0.p;
-    if (TC) sT[r][c] = T0;
-    if (VISC) {
-      svm[0][r][c] = um; svm[1][r][c] = vm; svm[2][r][c] = wm;
-      svp[0][r][c] = P1.u; svp[1][r][c] = P1.v; svp[2][r][c] = P1.w;
-    }
-    if (hown) {
-      sq[0][hr][hc] = H.r; sq[1][hr][hc] = H.u; sq[2][hr][hc] = H.v; sq[3][hr][hc] = H.w;
-      sq[4][hr][hc] = H.p;
-      if (TC) sT[hr][hc] = H.p / H.r;
-      if (VISC) {
-        svm[0][hr][hc] = Hm.u; svm[1][hr][hc] = Hm.v; svm[2][hr][hc] = Hm.w;
-        svp[0][hr][hc] = Hp.u; svp[1][hr][hc] = Hp.v; svp[2][hr][hc] = Hp.w;
-      }
-      if (k + 1 < k1) fetch_halo(k + 1, cz);  // consumed next plane
-    }
-    __syncthreads();
-
-    // the update's base state (and U^n of the RK2 combine) are fetched here,
-    // so their latency hides behind the face solves
-    const long long cell = (long long)gi + (long long)gj * P.sy + (long long)k * P.sz;
-    real ub[5], ur[5];
-    if (store) {
-#pragma unroll
-      for (int n = 0; n < 5; ++n) ub[n] = Uin[cell + n * P.sf];
-      if (Urk) {
-#pragma unroll
-        for (int n = 0; n < 5; ++n) ur[n] = Urk[cell + n * P.sf];
-      }
-    }
-
-    // ---- x faces: face between window columns c-1 and c for every interior row
-    if (interior) {
-      real F[5], po;
-      face_flux4<real, FV3D_IX, RECON, RS>(lds_prim(r, c - 2), lds_prim(r, c - 1), P0,
-                                           lds_prim(r, c + 1), P, F, po);
-#pragma unroll
-      for (int n = 0; n < 5; ++n) sfx[n][r - 2][lane] = F[n];
-    } else if (wave == NR - 1 && lane < TY) {
-      // the right face of column 63 for row lane+2
-      real F[5], po;
-      face_flux4<real, FV3D_IX, RECON, RS>(lds_prim(lane + 2, WX), lds_prim(lane + 2, WX + 1),
-                                           lds_prim(lane + 2, WX + 2),
-                                           lds_prim(lane + 2, WX + 3), P, F, po);
-#pragma unroll
-      for (int n = 0; n < 5; ++n) sfx[n][lane][WX] = F[n];
-    }
-    // ---- y faces: face between window rows r-1 and r
-    if (yworker) {
-      real F[5], po;
-      face_flux4<real, FV3D_IY, RECON, RS>(lds_prim(r - 2, c), lds_prim(r - 1, c), P0,
-                                           lds_prim(r + 1, c), P, F, po);
-#pragma unroll
-      for (int n = 0; n < 5; ++n) sfy[n][r - 2][lane] = F[n];
-    }
-    // ---- z face k+1/2 from registers
-    real Fzh[5] = {0, 0, 0, 0, 0}, pouth = 0;
-    Prim<real> sK1{};
-    if (interior) {
-      Prim<real> qL, qR;
-      if (RECON == FV3D_PLM) {
-        const Prim<real> sK{cz[0], cz[1], cz[2], cz[3], cz[4]};
-        sK1 = slope3<real, true>(P0, P1, P2);
-        qL = plus_face(P0, sK);
-        qR = minus_face(P1, sK1);
-      } else {
-        qL = P0;
-        qR = P1;
-        if (P.pcm_wb) {
-          qL.p = P0.p + P0.r * (real)P.g * (real)P.dz * (real)0.5;
-          qR.p = P1.p - P1.r * (real)P.g * (real)P.dz * (real)0.5;
-        }
-      }
-      riemann<real, RS>(rotate<real, FV3D_IZ>(qL), rotate<real, FV3D_IZ>(qR), (real)P.gamma, gm1,
-                        (real)P.entho, Fzh, pouth);
-      unrotate_flux<real, FV3D_IZ>(Fzh);
-    }
-    __syncthreads();
-
-    // ---- update (Update.h:145-160, then ThermalConduction.h, Viscosity.h)
-    if (store) {
-      real u[5];
-#pragma unroll
-      for (int n = 0; n < 5; ++n) u[n] = ub[n];
-#pragma unroll
-      for (int n = 0; n < 5; ++n) {
-        const real t = (sfx[n][r - 2][lane] - sfx[n][r - 2][lane + 1]) * dt;
-        u[n] = u[n] + (P.pow2x ? t * (real)P.idx : t / (real)P.dx);
-      }
-#pragma unroll
-      for (int n = 0; n < 5; ++n) {
-        const real t = (sfy[n][r - 2][lane] - sfy[n][r - 1][lane]) * dt;
-        u[n] = u[n] + (P.pow2y ? t * (real)P.idy : t / (real)P.dy);
-      }
-      real zl[5], zh[5];
-#pragma unroll
-      for (int n = 0; n < 5; ++n) { zl[n] = fz[n]; zh[n] = Fzh[n]; }
-      if (P.wb_flux_zbc && (k == 0 || k == P.Nz - 1)) {  // Update.h:132-137
-        if (k == 0) {
-          zl[0] = 0; zl[1] = 0; zl[2] = 0; zl[4] = 0;
-          zl[3] = pouth - P0.r * (real)P.g * (real)P.dz;
-        } else {
-          zh[0] = 0; zh[1] = 0; zh[2] = 0; zh[4] = 0;
-          zh[3] = pz + P0.r * (real)P.g * (real)P.dz;
-        }
-      } else if (P.c91_flux_zbc && (k == 0 || k == P.Nz - 1)) {  // Update.h:138-143
-        if (k == 0) { zl[0] = 0; zl[1] = 0; zl[2] = 0; zl[3] = pz; zl[4] = 0; }
-        else { zh[0] = 0; zh[1] = 0; zh[2] = 0; zh[3] = pouth; zh[4] = 0; }
-      }
-#pragma unroll
-      for (int n = 0; n < 5; ++n) {
-        const real t = (zl[n] - zh[n]) * dt;
-        u[n] = u[n] + (P.pow2z ? t * (real)P.idz : t / (real)P.dz);
-      }
-      if (P.gravity) {  // Update.h:148-151
-        u[3] = u[3] + dt * P0.r * (real)P.g;
-        u[4] = u[4] + dt * (real)0.5 * (zl[0] + zh[0]) * (real)P.g;
-      }
-      u[0] = fmax((real)1.0e-6, u[0]);  // Update.h:160
-
-      if (TC) {  // ThermalConduction.h:47-95, constant conductivity
-        const real TC_ = T0;
-        const real TL = sT[r][c - 1], TR = sT[r][c + 1];
-        const real TU = sT[r - 1][c], TD = sT[r + 1][c];
-        const real TF = Tm, TB = T1;
-        auto ddx = [&](real v) { return P.pow2x ? v * (real)P.idx : v / (real)P.dx; };
-        auto ddy = [&](real v) { return P.pow2y ? v * (real)P.idy : v / (real)P.dy; };
-        auto ddz = [&](real v) { return P.pow2z ? v * (real)P.idz : v / (real)P.dz; };
-        const real FL = ddx(kf * (TC_ - TL));
-        const real FR = ddx(kf * (TR - TC_));
-        const real FU = ddy(kf * (TC_ - TU));
-        const real FD = ddy(kf * (TD - TC_));
-        real FF = ddz(kf * (TC_ - TF));
-        real FB = ddz(kf * (TB - TC_));
-        if (k == 0 && P.bctc_zmin != FV3D_BCTC_NONE) {
-          if (P.bctc_zmin == FV3D_BCTC_FIXED_TEMPERATURE)
-            FF = ddz(kf * (real)2.0 * (TC_ - (real)P.bctc_zmin_value));
-          else if (P.bctc_zmin == FV3D_BCTC_FIXED_GRADIENT)
-            FF = kf * (real)P.bctc_zmin_value;
-        }
-        if (k == P.Nz - 1 && P.bctc_zmax != FV3D_BCTC_NONE) {
-          if (P.bctc_zmax == FV3D_BCTC_FIXED_TEMPERATURE)
-            FB = ddz(kf * (real)2.0 * ((real)P.bctc_zmax_value - TC_));
-          else if (P.bctc_zmax == FV3D_BCTC_FIXED_GRADIENT)
-            FB = kf * (real)P.bctc_zmax_value;
-        }
-        u[4] = u[4] + (dtdx * (FR - FL) + dtdy * (FD - FU) + dtdz * (FB - FF));
-      }
-
-      if (VISC) {  // Viscosity.h:28-153, stencil[dk][dj][di] of u, v, w
-        auto S = [&](int dk, int dj, int di, int v) -> real {
-          const int rr = r + dj - 1, cc = c + di - 1;
-          if (dk == 1) return sq[v][rr][cc];
-          if (dk == 0) return svm[v - 1][rr][cc];
-          return svp[v - 1][rr][cc];
-        };
-        const real four_thirds = (real)(4.0 / 3.0);
-        const real two_thirds = (real)(2.0 / 3.0);
-        real vx[5] = {0, 0, 0, 0, 0}, vy[5] = {0, 0, 0, 0, 0}, vz[5] = {0, 0, 0, 0, 0};
-#pragma unroll
-        for (int s = 1; s < 3; ++s) {
-          const real sgn = (s == 1 ? (real)-1.0 : (real)1.0);
-          {  // X
-            const real qiu = (real)0.5 * (S(1, 1, s, 1) + S(1, 1, s - 1, 1));
-            const real qiv = (real)0.5 * (S(1, 1, s, 2) + S(1, 1, s - 1, 2));
-            const real qiw = (real)0.5 * (S(1, 1, s, 3) + S(1, 1, s - 1, 3));
-            const real dudx = oodx * (S(1, 1, s, 1) - S(1, 1, s - 1, 1));
-            const real dvdx = oodx * (S(1, 1, s, 2) - S(1, 1, s - 1, 2));
-            const real dwdx = oodx * (S(1, 1, s, 3) - S(1, 1, s - 1, 3));
-            const real dudy = (real)0.25 * oody * (S(1, 2, s, 1) - S(1, 0, s, 1)
-                                                   + S(1, 2, s - 1, 1) - S(1, 0, s - 1, 1));
-            const real dvdy = (real)0.25 * oody * (S(1, 2, s, 2) - S(1, 0, s, 2)
-                                                   + S(1, 2, s - 1, 2) - S(1, 0, s - 1, 2));
-            const real dudz = (real)0.25 * oodz * (S(2, 1, s, 1) - S(0, 1, s, 1)
-                                                   + S(2, 1, s - 1, 1) - S(0, 1, s - 1, 1));
-            const real dwdz = (real)0.25 * oodz * (S(2, 1, s, 3) - S(0, 1, s, 3)
-                                                   + S(2, 1, s - 1, 3) - S(0, 1, s - 1, 3));
-            const real txx = four_thirds * dudx - two_thirds * (dvdy + dwdz);
-            const real txy = dvdx + dudy;
-            const real txz = dwdx + dudz;
-            vx[1] += sgn * mu * txx;
-            vx[2] += sgn * mu * txy;
-            vx[3] += sgn * mu * txz;
-            vx[4] += sgn * mu * (txx * qiu + txy * qiv + txz * qiw);
-          }
-          {  // Y
-            const real qiu = (real)0.5 * (S(1, s, 1, 1) + S(1, s - 1, 1, 1));
-            const real qiv = (real)0.5 * (S(1, s, 1, 2) + S(1, s - 1, 1, 2));
-            const real qiw = (real)0.5 * (S(1, s, 1, 3) + S(1, s - 1, 1, 3));
-            const real dudy = oody * (S(1, s, 1, 1) - S(1, s - 1, 1, 1));
-            const real dvdy = oody * (S(1, s, 1, 2) - S(1, s - 1, 1, 2));
-            const real dwdy = oody * (S(1, s, 1, 3) - S(1, s - 1, 1, 3));
-            const real dudx = (real)0.25 * oodx * (S(1, s, 2, 1) - S(1, s, 0, 1)
-                                                   + S(1, s - 1, 2, 1) - S(1, s - 1, 0, 1));
-            const real dvdx = (real)0.25 * oodx * (S(1, s, 2, 2) - S(1, s, 0, 2)
-                                                   + S(1, s - 1, 2, 2) - S(1, s - 1, 0, 2));
-            const real dwdz = (real)0.25 * oodz * (S(2, s, 1, 3) - S(0, s, 1, 3)
-                                                   + S(2, s - 1, 1, 3) - S(0, s - 1, 1, 3));
-            const real dvdz = (real)0.25 * oodz * (S(2, s, 1, 2) - S(0, s, 1, 2)
-                                                   + S(2, s - 1, 1, 2) - S(0, s - 1, 1, 2));
-            const real tyy = four_thirds * dvdy - two_thirds * (dudx + dwdz);
-            const real txy = dvdx + dudy;
-            const real tyz = dvdz + dwdy;
-            vy[1] += sgn * mu * txy;
-            vy[2] += sgn * mu * tyy;
-            vy[3] += sgn * mu * tyz;
-            vy[4] += sgn * mu * (txy * qiu + tyy * qiv + tyz * qiw);
-          }
-          {  // Z, with the reference's index and component mix-ups (Viscosity.h:122-133)
-            const real qiu = (real)0.5 * (S(s, 1, 1, 1) + S(s - 1, 1, 1, 1));
-            const real qiv = (real)0.5 * (S(s, 1, 1, 2) + S(s - 1, 1, 1, 2));
-            const real qiw = (real)0.5 * (S(s, 1, 1, 3) + S(s - 1, 1, 1, 3));
-            const real dudz = oodz * (S(s, 1, 1, 1) - S(s - 1, 1, 1, 1));
-            const real dvdz = oodz * (S(s, 1, 1, 2) - S(s - 1, 1, 1, 2));
-            const real dwdz = oodz * (S(s, 1, 1, 3) - S(s - 1, 1, 1, 3));
-            const real dudx = (real)0.25 * oodx * (S(s, 1, 2, 1) - S(1, s, 0, 1)
-                                                   + S(s - 1, 1, 2, 1) - S(1, s - 1, 0, 1));
-            const real dwdx = (real)0.25 * oodx * (S(s, 1, 2, 2) - S(1, s, 0, 3)
-                                                   + S(s - 1, 1, 2, 2) - S(1, s - 1, 0, 3));
-            const real dvdy = (real)0.25 * oodz * (S(s, 2, 1, 3) - S(s, 0, 1, 2)
-                                                   + S(s - 1, 2, 1, 3) - S(s - 1, 0, 1, 2));
-            const real dwdy = (real)0.25 * oodz * (S(s, 2, 1, 2) - S(s, 0, 1, 3)
-                                                   + S(s - 1, 2, 1, 2) - S(s - 1, 0, 1, 3));
-            const real tzz = four_thirds * dwdz - two_thirds * (dudx + dvdy);
-            const real txz = dudz + dwdx;
-            const real tyz = dvdz + dwdy;
-            vz[1] += sgn * mu * txz;
-            vz[2] += sgn * mu * tyz;
-            vz[3] += sgn * mu * tzz;
-            vz[4] += sgn * mu * (txz * qiu + tyz * qiv + tzz * qiw);
-          }
-        }
-#pragma unroll
-        for (int n = 1; n < 5; ++n) u[n] = u[n] + (vx[n] + vy[n] + vz[n]) * dt;
-      }
-
-      if (Urk) {  // Update.h:203
-#pragma unroll
-        for (int n = 0; n < 5; ++n) u[n] = (real)0.5 * (ur[n] + u[n]);
-      }
-#pragma unroll
-      for (int n = 0; n < 5; ++n) Uout[cell + n * P.sf] = u[n];
-
-      if (inv_next) {  // next step's inv_dt_hyp from the stored state (ComputeDt.h:32-35)
-        const Prim<double> q = cons_to_prim<double>(u[0], u[1], u[2], u[3], u[4], P.gm1);
-        const double cs = sound_speed(q.r, q.p, P.gamma);
-        const double a = cs + fabs(q.u), bb = cs + fabs(q.v), cc = cs + fabs(q.w);
-        const double ih = (P.pow2x ? a * P.idx : a / P.dx) + (P.pow2y ? bb * P.idy : bb / P.dy) +
-                          (P.pow2z ? cc * P.idz : cc / P.dz);
-        if (ih > inv_max) inv_max = ih;
-        if (!(ih <= 1.0e300)) bad = 1;  // NaN/inf: the reference's Max reducer drops NaN
-      }
-    }
-
-    // ---- roll the z window
-    if (VISC) { um = P0.u; vm = P0.v; wm = P0.w; }
-    if (TC) { Tm = T0; T0 = T1; T1 = T2; }
-    P0 = P1;
-    P1 = P2;
-    if (interior) {
-      if (RECON == FV3D_PLM) { cz[0] = sK1.r; cz[1] = sK1.u; cz[2] = sK1.v; cz[3] = sK1.w; cz[4] = sK1.p; }
-#pragma unroll
-      for (int n = 0; n < 5; ++n) fz[n] = Fzh[n];
-      pz = pouth;
-    }
-  }
-
-  if (inv_next) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      const double o = __shfl_xor(inv_max, off);
-      inv_max = (o > inv_max) ? o : inv_max;
-    }
-    if (lane == 0) sred[wave] = inv_max;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double m = 0.0;
-      for (int w = 0; w < NR; ++w) m = (sred[w] > m) ? sred[w] : m;
-      if (m > 0.0) atomicMax(inv_next, double_as_u64(m));
-    }
-    if (bad) atomicOr(health, 1);
-  }
-}
-
-// ---------------------------------------------------------------------------
-// launcher
-// ---------------------------------------------------------------------------
-template <typename real, int TY, int RECON, int RS, bool TC, bool VISC>
-static hipError_t launch_t(const StageLaunch<real>& L, hipStream_t s) {
-  DevParams P = L.P;
-  const int blocks = P.tiles_x * P.tiles_y * P.tiles_z;
-  hipLaunchKernelGGL((stage_kernel<real, TY, RECON, RS, TC, VISC>), dim3(blocks),
-                     dim3(64 * (TY + 4)), 0, s, L.Uin, L.Urk, L.Uout, L.inv_cur, L.inv_next,
-                     L.inv_zero, L.health, P);
-  return hipGetLastError();
-}
-
-template <typename real, int RECON, int RS>
-static hipError_t launch_phys(const StageLaunch<real>& L, hipStream_t s) {
-  const bool tc = L.tc, vi = L.visc;
-  if (tc && vi) return launch_t<real, STAGE_TY, RECON, RS, true, true>(L, s);
-  if (tc) return launch_t<real, STAGE_TY, RECON, RS, true, false>(L, s);
-  if (vi) return launch_t<real, STAGE_TY, RECON, RS, false, true>(L, s);
-  return launch_t<real, STAGE_TY, RECON, RS, false, false>(L, s);
-}
-
-template <typename real>
-hipError_t launch_stage(const StageLaunch<real>& L, hipStream_t s) {
-  const bool plm = L.recon == FV3D_PLM;
-  const bool hll = L.riemann == FV3D_HLL;
-  if (plm && hll) return launch_phys<real, FV3D_PLM, FV3D_HLL>(L, s);
-  if (plm) return launch_phys<real, FV3D_PLM, FV3D_HLLC>(L, s);
-  if (hll) return launch_phys<real, FV3D_PCM, FV3D_HLL>(L, s);
-  return launch_phys<real, FV3D_PCM, FV3D_HLLC>(L, s);
-}
-
-template hipError_t launch_stage<double>(const StageLaunch<double>&, hipStream_t);
 
 }  // namespace fv3d_hip

@@ -31,8 +31,9 @@ extern "C" {
 #define FV3D_ENOMEM 3      /* device allocation failed */
 #define FV3D_ENONFINITE 4  /* the state produced NaN/inf (pressure < 0, density 0, ...) */
 
-#define FV3D_FP64 0
-#define FV3D_FP32 1
+#define FV3D_FP64 0        /* bit-equal to the reference's FP64 arithmetic */
+#define FV3D_FP32 1        /* FP32 state and fluxes; RK2 accumulation Kahan-compensated */
+#define FV3D_FP32_PLAIN 2  /* FP32 state and fluxes, uncompensated */
 
 typedef struct fv3d_ctx fv3d_ctx;
 
@@ -44,7 +45,8 @@ const char* fv3d_last_error(const fv3d_ctx* ctx);
 /* Context = the state owned by the reference's main() (main.cpp:28-30) plus
  * the functors built from Params (main.cpp:38-41): UpdateFunctor
  * (Update.h:45), ComputeDtFunctor (ComputeDt.h:13), BoundaryManager
- * (BoundaryConditions.h:150).  precision is FV3D_FP64 or FV3D_FP32. */
+ * (BoundaryConditions.h:150).  precision is FV3D_FP64, FV3D_FP32 or
+ * FV3D_FP32_PLAIN; the dt reduction is FP64 in every precision. */
 int fv3d_create(const fv3d_params* p, int precision, int device, fv3d_ctx** out);
 int fv3d_destroy(fv3d_ctx* ctx);
 
