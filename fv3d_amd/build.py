@@ -60,5 +60,37 @@ def build_library(verbose: bool = False) -> str:
     return LIB
 
 
+DRIVER = os.path.join(HERE, "fv3d")
+DRIVER_SOURCES = ["main.cpp", "sim_info.cpp", "init.cpp", "io_manager.cpp"]
+HDF5_ROOT = os.environ.get("FV3D_HDF5_ROOT", "/opt/conda")
+
+
+def build_driver(verbose: bool = False) -> str:
+    """The fv3d executable: the reference's driver loop on libfv3d_hip, with
+    HDF5 snapshot output (HDF5 C library from FV3D_HDF5_ROOT)."""
+    hdf5_lib = os.path.join(HDF5_ROOT, "lib", "libhdf5.so")
+    # Only libhdf5 is taken from HDF5_ROOT: a private directory holds a link
+    # to it, so the rest of that prefix (an older libstdc++) stays off the
+    # executable's search path.
+    libdir = os.path.join(HERE, "_hdf5")
+    os.makedirs(libdir, exist_ok=True)
+    real = os.path.realpath(hdf5_lib)
+    link = os.path.join(libdir, os.path.basename(real).split(".so")[0] + ".so.103")
+    for name in os.listdir(libdir):
+        os.unlink(os.path.join(libdir, name))
+    os.symlink(os.path.join(HDF5_ROOT, "lib", os.path.basename(link)), link)
+    src = [os.path.join(HERE, "driver", f) for f in DRIVER_SOURCES]
+    tmp = DRIVER + ".tmp"
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", f"-I{INCLUDE}",
+           f"-I{os.path.join(HERE, 'driver')}", f"-I{os.path.join(HDF5_ROOT, 'include')}", *src,
+           "-o", tmp, LIB, hdf5_lib, "-Wl,-rpath,$ORIGIN:$ORIGIN/_hdf5"]
+    _run(cmd)
+    os.replace(tmp, DRIVER)
+    if verbose:
+        print("built", DRIVER)
+    return DRIVER
+
+
 if __name__ == "__main__":
     build_library(verbose=True)
+    build_driver(verbose=True)
