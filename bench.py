@@ -146,6 +146,58 @@ def form_line(F, p, Q0, precision, steps, warmup, alg_bytes):
             "roofline_frac": round(achieved / HBM_PEAK_GBS, 4)}
 
 
+def ensemble_form(F, nbox, steps, tune_cache=None):
+    """The batched ensemble: nbox Rayleigh-Taylor boxes (sides drawn from
+    {32,48,64,96} per axis, gamma ~ U[1.3,1.7], g=-0.1, periodic x/y,
+    reflecting z), PLM + HLLC, SSP-RK2, CFL 0.8, FP32 with Kahan-compensated
+    RK2.  Times `steps` steps from the initial condition with the persistent
+    kernels replayed from a hipGraph (launch shape from the autotuner), and
+    the same steps as a per-box launch loop; then runs the FP64 ensemble for
+    the same steps and reports L1(rho) of FP32 against FP64 per box."""
+    from fv3d_amd import problems
+    base = F.Params.defaults().replace(reconstruction=F.PLM, riemann_solver=F.HLLC,
+                                       time_stepping=F.TS_RK2, CFL=0.8)
+    ens = problems.rt_ensemble(base, nbox)
+    ps = [q for q, _ in ens]
+
+    def fresh(precision):
+        b = F.Batch(ps, precision)
+        for i, (_, Q0) in enumerate(ens):
+            b.set_prim(i, Q0)
+        return b
+
+    b32 = fresh(F.FP32)
+    cells = b32.cells
+    nzc, grid = b32.tune(nsteps=3, cache=tune_cache)
+    print(f"ensemble: tuned nzc={nzc} grid={grid}", file=sys.stderr, flush=True)
+    ms_graph = b32.time(steps, F.BATCH_GRAPH)
+    b32.compute_dt()  # raises on non-finite state
+    print(f"ensemble: graph {ms_graph:.1f} ms / {steps} steps", file=sys.stderr, flush=True)
+    bl = fresh(F.FP32)
+    ms_loop = bl.time(steps, F.BATCH_LOOP)
+    bl.close()
+    print(f"ensemble: loop {ms_loop:.1f} ms / {steps} steps", file=sys.stderr, flush=True)
+    b64 = fresh(F.FP64)
+    b64.step(b32.step_count, F.BATCH_GRAPH)  # the graph run's capture step is one more
+    b64.compute_dt()
+    b32_steps = b32.step_count
+    l1 = []
+    for i in range(nbox):
+        r32 = b32.get_prim(i, with_ghosts=False)[..., 0]
+        r64 = b64.get_prim(i, with_ghosts=False)[..., 0]
+        l1.append(float(np.mean(np.abs(r32 - r64))))
+    b32.close()
+    b64.close()
+    g = cells * steps / (ms_graph * 1e-3) / 1e9
+    lo = cells * steps / (ms_loop * 1e-3) / 1e9
+    return {"boxes": nbox, "cells": cells, "steps": steps, "l1_after_steps": b32_steps, "tuned_nzc": nzc, "tuned_grid": grid,
+            "graph_Gcells": round(g, 4), "loop_Gcells": round(lo, 4), "speedup_vs_loop": round(g / lo, 3),
+            "graph_ms_per_step": round(ms_graph / steps, 4), "loop_ms_per_step": round(ms_loop / steps, 4),
+            "alg_bytes_per_cell_update": 140.0,
+            "roofline_frac_graph": round(140.0 * g / HBM_PEAK_GBS, 4),
+            "l1_rho_fp32_vs_fp64_max": max(l1), "l1_rho_fp32_vs_fp64_mean": float(np.mean(l1))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -156,6 +208,8 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=256, help="cells per axis of the CPU-baseline sample")
     ap.add_argument("--stream", type=int, default=1, help="also measure the device copy bandwidth")
     ap.add_argument("--forms", type=int, default=1, help="also time the FP32 forms")
+    ap.add_argument("--ensemble", type=int, default=512, help="boxes of the batched ensemble (0: skip)")
+    ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--copy-only", action="store_true",
                     help="run only the 1 GiB copy (calibration run for the rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -221,6 +275,8 @@ def main():
             "fp32_kahan_rk2": form_line(F, p, Q0, F.FP32, args.steps, args.warmup, 140.0),
             "fp32_plain_rk2": form_line(F, p, Q0, F.FP32_PLAIN, args.steps, args.warmup, 100.0),
         }
+    if args.ensemble > 0:
+        out.setdefault("forms", {})["fp32_ensemble_rt"] = ensemble_form(F, args.ensemble, args.ensemble_steps)
     if args.cpu_budget > 0:
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_budget)
     else:

@@ -166,6 +166,19 @@ def lib() -> ctypes.CDLL:
         "fv3d_fill_boundaries": (ctypes.c_int, [P(Params), dp]),
         "fv3d_riemann": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_long, dp, dp, dp, dp]),
         "fv3d_stream_copy": (ctypes.c_int, [ctypes.c_size_t, ctypes.c_int, P(ctypes.c_float)]),
+        "fv3d_batch_create": (ctypes.c_int, [ctypes.c_int, P(Params), ctypes.c_int, ctypes.c_int, P(vp)]),
+        "fv3d_batch_destroy": (ctypes.c_int, [vp]),
+        "fv3d_batch_last_error": (ctypes.c_char_p, [vp]),
+        "fv3d_batch_set_prim": (ctypes.c_int, [vp, ctypes.c_int, dp, ctypes.c_int]),
+        "fv3d_batch_get_prim": (ctypes.c_int, [vp, ctypes.c_int, dp, ctypes.c_int]),
+        "fv3d_batch_compute_dt": (ctypes.c_int, [vp, dp]),
+        "fv3d_batch_step": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
+        "fv3d_batch_time": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, P(ctypes.c_float)]),
+        "fv3d_batch_step_count": (ctypes.c_long, [vp]),
+        "fv3d_batch_cells": (ctypes.c_longlong, [vp]),
+        "fv3d_batch_configure": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
+        "fv3d_batch_tune": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_char_p, P(ctypes.c_int),
+                                           P(ctypes.c_int)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -285,6 +298,92 @@ class Simulation:
             self._ctx, int(n), ctypes.byref(tot),
             st.ctypes.data_as(ctypes.POINTER(ctypes.c_float)) if per_stage else None), self._ctx)
         return tot.value, st
+
+
+BATCH_GRAPH, BATCH_PERSISTENT, BATCH_LOOP = 0, 1, 2
+
+
+class Batch:
+    """An ensemble of independent boxes advanced together (fv3d_batch_* in
+    include/fv3d_hip.h).  Each box is one reference run (main.cpp:51-76)
+    with its own Params and dt; the boxes must share reconstruction, solver,
+    time stepping and the conduction / viscosity switches."""
+
+    def __init__(self, params_list, precision: int = FP32, device: int = 0):
+        self.params = [q.copy() for q in params_list]
+        arr = (Params * len(self.params))(*self.params)
+        self._b = ctypes.c_void_p()
+        _check(lib().fv3d_batch_create(len(self.params), arr, precision, device, ctypes.byref(self._b)))
+
+    def _check(self, rc):
+        if rc != 0:
+            msg = lib().fv3d_batch_last_error(self._b)
+            raise FV3DError(rc, msg.decode() if msg else "")
+
+    def close(self):
+        if self._b:
+            lib().fv3d_batch_destroy(self._b)
+            self._b = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __len__(self):
+        return len(self.params)
+
+    def set_prim(self, box: int, Q: np.ndarray, with_ghosts: bool = True):
+        p = self.params[box]
+        shape = p.shape_ghosted if with_ghosts else p.shape_domain
+        Q = np.ascontiguousarray(Q, dtype=np.float64)
+        if Q.shape != shape:
+            raise ValueError(f"Q has shape {Q.shape}, expected {shape}")
+        self._check(lib().fv3d_batch_set_prim(self._b, box, _dptr(Q), int(with_ghosts)))
+
+    def get_prim(self, box: int, with_ghosts: bool = True) -> np.ndarray:
+        p = self.params[box]
+        out = np.zeros(p.shape_ghosted if with_ghosts else p.shape_domain)
+        self._check(lib().fv3d_batch_get_prim(self._b, box, _dptr(out), int(with_ghosts)))
+        return out
+
+    def compute_dt(self) -> np.ndarray:
+        dts = np.zeros(len(self.params))
+        self._check(lib().fv3d_batch_compute_dt(self._b, _dptr(dts)))
+        return dts
+
+    def step(self, n: int = 1, mode: int = BATCH_GRAPH):
+        self._check(lib().fv3d_batch_step(self._b, int(n), int(mode)))
+
+    def time(self, n: int, mode: int = BATCH_GRAPH) -> float:
+        ms = ctypes.c_float()
+        self._check(lib().fv3d_batch_time(self._b, int(n), int(mode), ctypes.byref(ms)))
+        return ms.value
+
+    @property
+    def step_count(self) -> int:
+        return int(lib().fv3d_batch_step_count(self._b))
+
+    @property
+    def cells(self) -> int:
+        return int(lib().fv3d_batch_cells(self._b))
+
+    def configure(self, nzc_cap: int = 0, grid: int = 0):
+        self._check(lib().fv3d_batch_configure(self._b, int(nzc_cap), int(grid)))
+
+    def tune(self, nsteps: int = 3, cache: Optional[str] = None):
+        """Autotune the persistent launch shape; returns (nzc_cap, grid)."""
+        nzc, grid = ctypes.c_int(), ctypes.c_int()
+        self._check(lib().fv3d_batch_tune(self._b, int(nsteps), cache.encode() if cache else None,
+                                          ctypes.byref(nzc), ctypes.byref(grid)))
+        return nzc.value, grid.value
 
 
 def fill_boundaries(params: Params, Q: np.ndarray) -> np.ndarray:

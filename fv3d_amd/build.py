@@ -29,9 +29,13 @@ HIP_FLAGS = [
     "-Wno-unused-value", "-Wno-unused-result", f"-I{INCLUDE}", f"-I{CSRC}",
 ]
 
-SOURCES = ["fv3d_stage.hip", "fv3d_stage_f32.hip", "fv3d_aux.hip", "fv3d_api.hip"]
+SOURCES = ["fv3d_stage.hip", "fv3d_stage_f32.hip", "fv3d_batch.hip", "fv3d_batch_f32.hip",
+           "fv3d_aux.hip", "fv3d_api.hip", "fv3d_batch_api.hip"]
 # per-source extra flags
-EXTRA = {"fv3d_stage_f32.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt"]}
+EXTRA = {
+    "fv3d_stage_f32.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt"],
+    "fv3d_batch_f32.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt"],
+}
 
 
 def _run(cmd):

@@ -1,0 +1,542 @@
+// fv3d_batch_api.hip -- C ABI for ensembles of independent boxes
+// (declared in include/fv3d_hip.h, fv3d_batch_*).
+//
+// All boxes share the numerics that select kernel instances (reconstruction,
+// Riemann solver, time stepping, conduction and viscosity on/off, precision);
+// everything else -- sizes, gamma, boundary types, gravity, coefficients,
+// CFL -- is per box.  Each box has its own dt.
+//
+// Steps run in one of three ways:
+//   FV3D_BATCH_GRAPH       one persistent launch per RK stage plus the step
+//                          advance, captured once in a hipGraph and replayed;
+//   FV3D_BATCH_PERSISTENT  the same launches issued from the host each step;
+//   FV3D_BATCH_LOOP        one launch per box per stage (the per-instance loop).
+#include <algorithm>
+#include <cstdio>
+#include <fstream>
+#include <map>
+#include <numeric>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "fv3d_hip.h"
+#include "fv3d_host.h"
+#include "fv3d_kernels.h"
+
+using namespace fv3d_hip;
+
+struct fv3d_batch {
+  int nbox = 0;
+  int precision = FV3D_FP64;
+  int device = 0;
+  std::vector<fv3d_params> params;
+  std::vector<DevParams> dp;       // per box, z chunks capped for the work list
+  std::vector<DevParams> dp_loop;  // per box, the single-box launch shape
+  std::vector<long long> off;  // element offsets
+  long long total = 0;         // elements per state buffer
+  size_t elem = 8;
+  hipStream_t stream = nullptr;
+  void *A = nullptr, *B = nullptr, *C = nullptr, *comp = nullptr;
+  unsigned long long* rings = nullptr;  // 4 slots per box
+  int* health = nullptr;
+  long long* step_dev = nullptr;
+  unsigned* counters = nullptr;  // 16: 8 per stage
+  BoxDesc* boxes_dev = nullptr;
+  int2* items_dev = nullptr;
+  int* qstart_dev = nullptr;
+  int nitems = 0;
+  int grid = 0;
+  int nzc_cap = 0;  // z-chunk cap (0: per-box default)
+  long step = 0;    // host mirror of the device step counter
+  double* qbuf = nullptr;
+  size_t qbuf_elems = 0;
+  hipGraphExec_t graph = nullptr;
+  int num_cus = 256;
+  std::string err;
+};
+
+static int bfail(fv3d_batch* b, int code, const std::string& msg) {
+  if (b) b->err = msg;
+  return record_error(code, msg);
+}
+
+#define BCHK(b, expr)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return bfail((b), FV3D_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+static bool bkahan(const fv3d_batch* b) {
+  return b->precision == FV3D_FP32 && b->params[0].time_stepping == FV3D_TS_RK2;
+}
+
+// Builds per-box kernel parameters and the work list: boxes are packed onto
+// the 8 XCD queues by total cost (longest first), and each queue's items
+// are ordered longest first.
+static int build_worklist(fv3d_batch* b) {
+  const int n = b->nbox;
+  b->dp.resize(n);
+  b->dp_loop.resize(n);
+  std::vector<BoxDesc> desc(n);
+  std::vector<double> box_cost(n);
+  std::vector<std::vector<std::pair<double, int2>>> box_items(n);
+  for (int i = 0; i < n; ++i) {
+    const fv3d_params& p = b->params[i];
+    int nzc = b->nzc_cap;
+    if (nzc <= 0) nzc = std::min(p.Nz, 32);
+    b->dp[i] = make_devparams(p, nzc);
+    b->dp_loop[i] = make_devparams(p);
+    DevParams& d = b->dp[i];
+    desc[i].P = d;
+    desc[i].off = b->off[i];
+    desc[i].ring = b->rings + 4 * i;
+    const int ntiles = d.tiles_x * d.tiles_y * d.tiles_z;
+    for (int t = 0; t < ntiles; ++t) {
+      const int tz = t / (d.tiles_x * d.tiles_y);
+      const int planes = std::min(d.nzc, p.Nz - tz * d.nzc) + 4;
+      box_items[i].push_back({(double)planes, int2{i, t}});
+      box_cost[i] += planes;
+    }
+  }
+  std::vector<int> order(n);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return box_cost[a] > box_cost[c]; });
+  double load[8] = {0};
+  std::vector<std::pair<double, int2>> q[8];
+  for (int i : order) {
+    const int x = (int)(std::min_element(load, load + 8) - load);
+    load[x] += box_cost[i];
+    q[x].insert(q[x].end(), box_items[i].begin(), box_items[i].end());
+  }
+  std::vector<int2> items;
+  std::vector<int> qstart(9, 0);
+  for (int x = 0; x < 8; ++x) {
+    std::stable_sort(q[x].begin(), q[x].end(),
+                     [](const std::pair<double, int2>& a, const std::pair<double, int2>& c) { return a.first > c.first; });
+    qstart[x] = (int)items.size();
+    for (auto& it : q[x]) items.push_back(it.second);
+  }
+  qstart[8] = (int)items.size();
+  b->nitems = (int)items.size();
+  if (b->boxes_dev) hipFree(b->boxes_dev);
+  if (b->items_dev) hipFree(b->items_dev);
+  if (b->qstart_dev) hipFree(b->qstart_dev);
+  b->boxes_dev = nullptr; b->items_dev = nullptr; b->qstart_dev = nullptr;
+  BCHK(b, hipMalloc(&b->boxes_dev, n * sizeof(BoxDesc)));
+  BCHK(b, hipMalloc(&b->items_dev, items.size() * sizeof(int2)));
+  BCHK(b, hipMalloc(&b->qstart_dev, 9 * sizeof(int)));
+  BCHK(b, hipMemcpy(b->boxes_dev, desc.data(), n * sizeof(BoxDesc), hipMemcpyHostToDevice));
+  BCHK(b, hipMemcpy(b->items_dev, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice));
+  BCHK(b, hipMemcpy(b->qstart_dev, qstart.data(), 9 * sizeof(int), hipMemcpyHostToDevice));
+  if (b->graph) {
+    hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  return FV3D_OK;
+}
+
+// device buffer holding box i's U^n at the current step
+static void* box_state(fv3d_batch* b, int i) {
+  void* base = b->A;
+  if (bkahan(b)) base = (b->step & 1) ? b->C : b->A;
+  else if (b->params[0].time_stepping == FV3D_TS_EULER) base = (b->step & 1) ? b->B : b->A;
+  return (char*)base + b->off[i] * b->elem;
+}
+
+extern "C" {
+
+int fv3d_batch_create(int nbox, const fv3d_params* params, int precision, int device,
+                      fv3d_batch** out) {
+  if (nbox <= 0 || !params || !out) return record_error(FV3D_EINVAL, "bad argument");
+  *out = nullptr;
+  if (precision != FV3D_FP64 && precision != FV3D_FP32 && precision != FV3D_FP32_PLAIN)
+    return record_error(FV3D_EINVAL, "precision not supported");
+  const fv3d_params& p0 = params[0];
+  for (int i = 0; i < nbox; ++i) {
+    std::string why;
+    if (check_params(params[i], why)) return record_error(FV3D_EINVAL, "box " + std::to_string(i) + ": " + why);
+    const fv3d_params& p = params[i];
+    if (p.reconstruction != p0.reconstruction || p.riemann_solver != p0.riemann_solver ||
+        p.time_stepping != p0.time_stepping ||
+        p.thermal_conductivity_active != p0.thermal_conductivity_active ||
+        p.viscosity_active != p0.viscosity_active)
+      return record_error(FV3D_EINVAL, "boxes must share reconstruction, solver, time stepping, "
+                                       "conduction and viscosity switches");
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return record_error(FV3D_EHIP, "no HIP device visible");
+  if (device < 0 || device >= ndev) return record_error(FV3D_EINVAL, "device index out of range");
+  fv3d_batch* b = new fv3d_batch();
+  b->nbox = nbox;
+  b->precision = precision;
+  b->device = device;
+  b->params.assign(params, params + nbox);
+  b->elem = precision == FV3D_FP64 ? sizeof(double) : sizeof(float);
+  long long o = 0;
+  for (int i = 0; i < nbox; ++i) {
+    b->off.push_back(o);
+    const long long n = 5LL * params[i].Nx * params[i].Ny * params[i].Nz;
+    o += (n + 63) / 64 * 64;  // 256-byte aligned boxes
+  }
+  b->total = o;
+  auto cleanup = [&](int code, const std::string& m) {
+    fv3d_batch_destroy(b);
+    return record_error(code, m);
+  };
+  if (hipSetDevice(device) != hipSuccess) return cleanup(FV3D_EHIP, "hipSetDevice failed");
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+    b->num_cus = prop.multiProcessorCount;
+  b->grid = b->num_cus;
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+    return cleanup(FV3D_EHIP, "hipStreamCreate failed");
+  const size_t bytes = (size_t)b->total * b->elem;
+  if (hipMalloc(&b->A, bytes) != hipSuccess || hipMalloc(&b->B, bytes) != hipSuccess)
+    return cleanup(FV3D_ENOMEM, "state allocation failed");
+  if (bkahan(b) && (hipMalloc(&b->C, bytes) != hipSuccess || hipMalloc(&b->comp, bytes) != hipSuccess))
+    return cleanup(FV3D_ENOMEM, "state allocation failed");
+  if (hipMalloc(&b->rings, 4 * nbox * sizeof(unsigned long long)) != hipSuccess ||
+      hipMalloc(&b->health, sizeof(int)) != hipSuccess ||
+      hipMalloc(&b->step_dev, sizeof(long long)) != hipSuccess ||
+      hipMalloc(&b->counters, 16 * sizeof(unsigned)) != hipSuccess)
+    return cleanup(FV3D_ENOMEM, "allocation failed");
+  if (hipMemset(b->rings, 0, 4 * nbox * sizeof(unsigned long long)) != hipSuccess ||
+      hipMemset(b->health, 0, sizeof(int)) != hipSuccess ||
+      hipMemset(b->step_dev, 0, sizeof(long long)) != hipSuccess ||
+      hipMemset(b->counters, 0, 16 * sizeof(unsigned)) != hipSuccess ||
+      hipMemset(b->A, 0, bytes) != hipSuccess)
+    return cleanup(FV3D_EHIP, "memset failed");
+  if (b->comp && hipMemset(b->comp, 0, bytes) != hipSuccess) return cleanup(FV3D_EHIP, "memset failed");
+  if (int rc = build_worklist(b)) return cleanup(rc, b->err);
+  *out = b;
+  return FV3D_OK;
+}
+
+int fv3d_batch_destroy(fv3d_batch* b) {
+  if (!b) return FV3D_OK;
+  hipSetDevice(b->device);
+  if (b->stream) hipStreamSynchronize(b->stream);
+  if (b->graph) hipGraphExecDestroy(b->graph);
+  for (void* p : {b->A, b->B, b->C, b->comp, (void*)b->rings, (void*)b->health, (void*)b->step_dev,
+                  (void*)b->counters, (void*)b->boxes_dev, (void*)b->items_dev, (void*)b->qstart_dev,
+                  (void*)b->qbuf})
+    if (p) hipFree(p);
+  if (b->stream) hipStreamDestroy(b->stream);
+  delete b;
+  return FV3D_OK;
+}
+
+const char* fv3d_batch_last_error(const fv3d_batch* b) { return b ? b->err.c_str() : ""; }
+
+int fv3d_batch_set_prim(fv3d_batch* b, int box, const double* Q, int with_ghosts) {
+  if (!b || !Q || box < 0 || box >= b->nbox) return bfail(b, FV3D_EINVAL, "bad argument");
+  BCHK(b, hipSetDevice(b->device));
+  const fv3d_params& p = b->params[box];
+  std::vector<double> h;
+  aos_to_soa_domain(p, Q, with_ghosts, h);
+  double* d = nullptr;
+  BCHK(b, hipMalloc(&d, h.size() * sizeof(double)));
+  void* U = box_state(b, box);
+  hipError_t e = hipMemcpyAsync(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, b->stream);
+  if (e == hipSuccess)
+    e = b->precision == FV3D_FP64 ? launch_prim_to_cons<double>(d, (double*)U, b->dp[box], b->stream)
+                                  : launch_prim_to_cons<float>(d, (float*)U, b->dp[box], b->stream);
+  unsigned long long* ring = b->rings + 4 * box;
+  if (e == hipSuccess) e = launch_fill_zero_u64(ring, 4, b->stream);
+  if (e == hipSuccess)
+    e = b->precision == FV3D_FP64
+            ? launch_dt_reduce<double>((const double*)U, ring + (b->step % 3), b->health, b->dp[box], b->stream)
+            : launch_dt_reduce<float>((const float*)U, ring + (b->step % 3), b->health, b->dp[box], b->stream);
+  if (e == hipSuccess && b->comp)
+    e = hipMemsetAsync((char*)b->comp + b->off[box] * b->elem, 0,
+                       5ull * p.Nx * p.Ny * p.Nz * b->elem, b->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+  hipFree(d);
+  if (e != hipSuccess) return bfail(b, FV3D_EHIP, std::string("batch_set_prim: ") + hipGetErrorString(e));
+  return FV3D_OK;
+}
+
+int fv3d_batch_get_prim(fv3d_batch* b, int box, double* Q, int with_ghosts) {
+  if (!b || !Q || box < 0 || box >= b->nbox) return bfail(b, FV3D_EINVAL, "bad argument");
+  BCHK(b, hipSetDevice(b->device));
+  const fv3d_params& p = b->params[box];
+  const size_t nt = (size_t)p.Ntx * p.Nty * p.Ntz;
+  if (b->qbuf_elems < 5 * nt) {
+    if (b->qbuf) hipFree(b->qbuf);
+    b->qbuf = nullptr;
+    BCHK(b, hipMalloc(&b->qbuf, 5 * nt * sizeof(double)));
+    b->qbuf_elems = 5 * nt;
+  }
+  void* U = box_state(b, box);
+  if (b->precision == FV3D_FP64)
+    BCHK(b, launch_cons_to_prim_ghosted<double>((const double*)U, nullptr, b->qbuf, b->dp[box], p.Ng, b->stream));
+  else
+    BCHK(b, launch_cons_to_prim_ghosted<float>((const float*)U, nullptr, b->qbuf, b->dp[box], p.Ng, b->stream));
+  if (with_ghosts) BCHK(b, launch_fill_boundaries(b->qbuf, p, b->stream));
+  std::vector<double> h(5 * nt);
+  BCHK(b, hipMemcpyAsync(h.data(), b->qbuf, 5 * nt * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  BCHK(b, hipStreamSynchronize(b->stream));
+  if (with_ghosts) {
+    soa_ghosted_to_aos(p, h.data(), Q);
+  } else {
+    for (int k = 0; k < p.Nz; ++k)
+      for (int j = 0; j < p.Ny; ++j)
+        for (int i = 0; i < p.Nx; ++i) {
+          const size_t s = (i + p.Ng) + (size_t)p.Ntx * ((j + p.Ng) + (size_t)p.Nty * (k + p.Ng));
+          double* d = Q + (((size_t)k * p.Ny + j) * p.Nx + i) * 5;
+          for (int f = 0; f < 5; ++f) d[f] = h[f * nt + s];
+        }
+  }
+  return FV3D_OK;
+}
+
+int fv3d_batch_compute_dt(fv3d_batch* b, double* dts) {
+  if (!b || !dts) return bfail(b, FV3D_EINVAL, "bad argument");
+  BCHK(b, hipSetDevice(b->device));
+  std::vector<unsigned long long> r(4 * b->nbox);
+  int health = 0;
+  BCHK(b, hipMemcpyAsync(r.data(), b->rings, r.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, b->stream));
+  BCHK(b, hipMemcpyAsync(&health, b->health, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+  BCHK(b, hipStreamSynchronize(b->stream));
+  for (int i = 0; i < b->nbox; ++i) {
+    double ih;
+    std::memcpy(&ih, &r[4 * i + (b->step % 3)], sizeof ih);
+    double m = ih;
+    if (m < b->dp[i].inv_tc) m = b->dp[i].inv_tc;
+    if (m < b->dp[i].inv_visc) m = b->dp[i].inv_visc;
+    dts[i] = b->params[i].CFL / m;
+  }
+  if (health) return bfail(b, FV3D_ENONFINITE, "non-finite inverse time step in the ensemble");
+  return FV3D_OK;
+}
+
+}  // extern "C"
+
+template <typename real>
+static int enqueue_persistent(fv3d_batch* b) {
+  const fv3d_params& p0 = b->params[0];
+  BatchLaunch L;
+  L.boxes = b->boxes_dev;
+  L.items = b->items_dev;
+  L.qstart = b->qstart_dev;
+  L.step = b->step_dev;
+  L.bufs = BatchBufs{b->A, b->B, b->C, b->comp, b->health};
+  L.ts = p0.time_stepping;
+  L.grid = b->grid;
+  L.recon = p0.reconstruction == FV3D_PLM ? FV3D_PLM : FV3D_PCM;
+  L.riemann = p0.riemann_solver;
+  L.tc = p0.thermal_conductivity_active != 0;
+  L.visc = p0.viscosity_active != 0;
+  const bool rk2 = p0.time_stepping == FV3D_TS_RK2;
+  const bool k = bkahan(b);
+  L.mode = k ? 1 : 0;
+  L.stage = 0;
+  L.last_stage = rk2 ? 0 : 1;
+  L.counters = b->counters;
+  BCHK(b, launch_batch_stage<real>(L, b->stream));
+  if (rk2) {
+    L.mode = k ? 2 : 0;
+    L.stage = 1;
+    L.last_stage = 1;
+    L.counters = b->counters + 8;
+    BCHK(b, launch_batch_stage<real>(L, b->stream));
+  }
+  BCHK(b, launch_batch_advance(b->step_dev, b->counters, 16, b->stream));
+  return FV3D_OK;
+}
+
+// one launch per box per stage, with the box's own kernel parameters
+template <typename real>
+static int enqueue_loop(fv3d_batch* b) {
+  const fv3d_params& p0 = b->params[0];
+  const bool rk2 = p0.time_stepping == FV3D_TS_RK2;
+  const bool k = bkahan(b);
+  const long s = b->step;
+  for (int i = 0; i < b->nbox; ++i) {
+    StageLaunch<real> L;
+    L.recon = p0.reconstruction == FV3D_PLM ? FV3D_PLM : FV3D_PCM;
+    L.riemann = p0.riemann_solver;
+    L.tc = p0.thermal_conductivity_active != 0;
+    L.visc = p0.viscosity_active != 0;
+    L.health = b->health;
+    L.P = b->dp_loop[i];
+    unsigned long long* ring = b->rings + 4 * i;
+    L.inv_cur = ring + (s % 3);
+    real* A = (real*)b->A + b->off[i];
+    real* B = (real*)b->B + b->off[i];
+    if (k) {
+      real* C = (real*)b->C + b->off[i];
+      real* cur = (s & 1) ? C : A;
+      real* nxt = (s & 1) ? A : C;
+      L.mode = 1; L.Uin = cur; L.Urk = nullptr; L.Uout = B; L.inv_next = nullptr; L.inv_zero = ring + ((s + 2) % 3);
+      BCHK(b, launch_stage<real>(L, b->stream));
+      L.mode = 2; L.Uin = cur; L.Kin = B; L.Cio = (real*)b->comp + b->off[i]; L.Uout = nxt;
+      L.inv_next = ring + ((s + 1) % 3); L.inv_zero = nullptr;
+      BCHK(b, launch_stage<real>(L, b->stream));
+    } else if (rk2) {
+      L.Uin = A; L.Urk = nullptr; L.Uout = B; L.inv_next = nullptr; L.inv_zero = ring + ((s + 2) % 3);
+      BCHK(b, launch_stage<real>(L, b->stream));
+      L.Uin = B; L.Urk = A; L.Uout = A; L.inv_next = ring + ((s + 1) % 3); L.inv_zero = nullptr;
+      BCHK(b, launch_stage<real>(L, b->stream));
+    } else {
+      L.Uin = (s & 1) ? B : A; L.Uout = (s & 1) ? A : B;
+      L.inv_next = ring + ((s + 1) % 3); L.inv_zero = ring + ((s + 2) % 3);
+      BCHK(b, launch_stage<real>(L, b->stream));
+    }
+  }
+  BCHK(b, launch_batch_advance(b->step_dev, b->counters, 16, b->stream));
+  return FV3D_OK;
+}
+
+static int enqueue_step(fv3d_batch* b, int mode) {
+  if (mode == FV3D_BATCH_LOOP)
+    return b->precision == FV3D_FP64 ? enqueue_loop<double>(b) : enqueue_loop<float>(b);
+  if (mode == FV3D_BATCH_GRAPH) {
+    if (!b->graph) {
+      hipGraph_t g;
+      BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      const int rc = b->precision == FV3D_FP64 ? enqueue_persistent<double>(b) : enqueue_persistent<float>(b);
+      hipError_t e = hipStreamEndCapture(b->stream, &g);
+      if (rc) return rc;
+      BCHK(b, e);
+      e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+      hipGraphDestroy(g);
+      BCHK(b, e);
+    }
+    BCHK(b, hipGraphLaunch(b->graph, b->stream));
+    return FV3D_OK;
+  }
+  return b->precision == FV3D_FP64 ? enqueue_persistent<double>(b) : enqueue_persistent<float>(b);
+}
+
+extern "C" {
+
+int fv3d_batch_step(fv3d_batch* b, int nsteps, int mode) {
+  if (!b || nsteps < 0) return bfail(b, FV3D_EINVAL, "bad argument");
+  BCHK(b, hipSetDevice(b->device));
+  for (int i = 0; i < nsteps; ++i) {
+    if (int rc = enqueue_step(b, mode)) return rc;
+    ++b->step;
+  }
+  return FV3D_OK;
+}
+
+int fv3d_batch_time(fv3d_batch* b, int nsteps, int mode, float* ms) {
+  if (!b || nsteps <= 0 || !ms) return bfail(b, FV3D_EINVAL, "bad argument");
+  BCHK(b, hipSetDevice(b->device));
+  hipEvent_t e0, e1;
+  BCHK(b, hipEventCreate(&e0));
+  BCHK(b, hipEventCreate(&e1));
+  if (mode == FV3D_BATCH_GRAPH && !b->graph) {  // capture outside the timed window
+    if (int rc = enqueue_step(b, mode)) return rc;
+    ++b->step;
+  }
+  BCHK(b, hipEventRecord(e0, b->stream));
+  int rc = FV3D_OK;
+  for (int i = 0; i < nsteps && rc == FV3D_OK; ++i) {
+    rc = enqueue_step(b, mode);
+    ++b->step;
+  }
+  BCHK(b, hipEventRecord(e1, b->stream));
+  BCHK(b, hipEventSynchronize(e1));
+  BCHK(b, hipEventElapsedTime(ms, e0, e1));
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  return rc;
+}
+
+long fv3d_batch_step_count(const fv3d_batch* b) { return b ? b->step : -1; }
+
+long long fv3d_batch_cells(const fv3d_batch* b) {
+  if (!b) return -1;
+  long long n = 0;
+  for (const auto& p : b->params) n += (long long)p.Nx * p.Ny * p.Nz;
+  return n;
+}
+
+int fv3d_batch_configure(fv3d_batch* b, int nzc_cap, int grid) {
+  if (!b || nzc_cap < 0 || grid < 0) return bfail(b, FV3D_EINVAL, "bad argument");
+  BCHK(b, hipSetDevice(b->device));
+  BCHK(b, hipStreamSynchronize(b->stream));
+  b->nzc_cap = nzc_cap;
+  b->grid = grid > 0 ? grid : b->num_cus;
+  return build_worklist(b);
+}
+
+int fv3d_batch_tune(fv3d_batch* b, int nsteps, const char* cache_path, int* nzc_out, int* grid_out) {
+  if (!b || nsteps <= 0) return bfail(b, FV3D_EINVAL, "bad argument");
+  BCHK(b, hipSetDevice(b->device));
+  // signature of the ensemble: shapes, precision and physics switches
+  std::map<std::string, int> shapes;
+  for (const auto& p : b->params)
+    shapes[std::to_string(p.Nx) + "x" + std::to_string(p.Ny) + "x" + std::to_string(p.Nz)]++;
+  const fv3d_params& p0 = b->params[0];
+  std::ostringstream sig;
+  sig << "prec" << b->precision << "_r" << p0.reconstruction << "_s" << p0.riemann_solver << "_t"
+      << p0.time_stepping << "_c" << p0.thermal_conductivity_active << "_v" << p0.viscosity_active;
+  for (auto& kv : shapes) sig << "_" << kv.first << ":" << kv.second;
+  const std::string key = sig.str();
+  if (cache_path && *cache_path) {
+    std::ifstream in(cache_path);
+    std::string k;
+    int nzc, grid;
+    while (in >> k >> nzc >> grid)
+      if (k == key) {
+        if (nzc_out) *nzc_out = nzc;
+        if (grid_out) *grid_out = grid;
+        return fv3d_batch_configure(b, nzc, grid);
+      }
+  }
+  // candidates: z-chunk cap x persistent grid size, timed on a copy of the state
+  const size_t bytes = (size_t)b->total * b->elem;
+  std::vector<void*> saved;
+  for (void* p : {b->A, b->B, b->C, b->comp}) {
+    void* s = nullptr;
+    if (p) {
+      BCHK(b, hipMalloc(&s, bytes));
+      BCHK(b, hipMemcpyAsync(s, p, bytes, hipMemcpyDeviceToDevice, b->stream));
+    }
+    saved.push_back(s);
+  }
+  std::vector<unsigned long long> rings(4 * b->nbox);
+  long long step_dev = 0;
+  BCHK(b, hipMemcpyAsync(rings.data(), b->rings, rings.size() * 8, hipMemcpyDeviceToHost, b->stream));
+  BCHK(b, hipMemcpyAsync(&step_dev, b->step_dev, 8, hipMemcpyDeviceToHost, b->stream));
+  BCHK(b, hipStreamSynchronize(b->stream));
+  const long step0 = b->step;
+  auto restore = [&]() -> int {
+    void* bufs[4] = {b->A, b->B, b->C, b->comp};
+    for (int i = 0; i < 4; ++i)
+      if (bufs[i]) BCHK(b, hipMemcpyAsync(bufs[i], saved[i], bytes, hipMemcpyDeviceToDevice, b->stream));
+    BCHK(b, hipMemcpyAsync(b->rings, rings.data(), rings.size() * 8, hipMemcpyHostToDevice, b->stream));
+    BCHK(b, hipMemcpyAsync(b->step_dev, &step_dev, 8, hipMemcpyHostToDevice, b->stream));
+    BCHK(b, hipStreamSynchronize(b->stream));
+    b->step = step0;
+    return FV3D_OK;
+  };
+  int best_nzc = 0, best_grid = b->num_cus;
+  float best = 1e30f;
+  for (int nzc : {8, 16, 32, 64}) {
+    for (int mult : {1, 2, 4}) {
+      const int grid = b->num_cus * mult;
+      if (int rc = fv3d_batch_configure(b, nzc, grid)) return rc;
+      float ms = 0.f;
+      if (int rc = fv3d_batch_time(b, nsteps, FV3D_BATCH_GRAPH, &ms)) return rc;
+      if (int rc = restore()) return rc;
+      if (ms < best) { best = ms; best_nzc = nzc; best_grid = grid; }
+    }
+  }
+  for (void* s : saved)
+    if (s) hipFree(s);
+  if (cache_path && *cache_path) {
+    std::ofstream outf(cache_path, std::ios::app);
+    outf << key << " " << best_nzc << " " << best_grid << "\n";
+  }
+  if (nzc_out) *nzc_out = best_nzc;
+  if (grid_out) *grid_out = best_grid;
+  return fv3d_batch_configure(b, best_nzc, best_grid);
+}
+
+}  // extern "C"

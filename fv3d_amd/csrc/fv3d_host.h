@@ -1,0 +1,135 @@
+// fv3d_host.h -- host-side helpers shared by the single-box and batch C ABI:
+// parameter checks, kernel parameters, and the host layout transposes.
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "fv3d_hip.h"
+#include "fv3d_kernels.h"
+
+namespace fv3d_hip {
+
+// records msg as the library-wide last error (fv3d_last_error(NULL)); returns code
+int record_error(int code, const std::string& msg);
+
+inline bool is_pow2(double d) {
+  if (!(d > 0.0) || !std::isfinite(d)) return false;
+  int e;
+  return std::frexp(d, &e) == 0.5;
+}
+
+// inv_dt_par for a constant coefficient (ComputeDt.h:37-47)
+inline double inv_dt_par(bool active, double coef, const fv3d_params& p) {
+  if (!active) return p.epsilon;
+  return std::fmax(std::fmax(2.0 * coef / (p.dx * p.dx), 2.0 * coef / (p.dy * p.dy)),
+                   2.0 * coef / (p.dz * p.dz));
+}
+
+inline int tiles_z_for(const fv3d_params& p, int tiles_xy) {
+  int nzc = 0;
+  if (const char* e = std::getenv("FV3D_NZC")) nzc = std::atoi(e);
+  if (nzc <= 0) {
+    const int target = 512;  // >= 2 workgroups per CU on 256 CUs
+    int tz = (target + tiles_xy - 1) / tiles_xy;
+    tz = std::max(1, std::min(tz, p.Nz / 8 > 0 ? p.Nz / 8 : 1));
+    nzc = (p.Nz + tz - 1) / tz;
+  }
+  return std::max(1, std::min(nzc, p.Nz));
+}
+
+inline int check_params(const fv3d_params& p, std::string& why) {
+  if (p.Nx < 2 || p.Ny < 2 || p.Nz < 2) { why = "Nx, Ny, Nz must be >= 2"; return 1; }
+  if (p.Ng < (p.reconstruction == FV3D_PLM ? 2 : 1)) { why = "too few ghost layers"; return 1; }
+  if (p.boundary_z == FV3D_BC_C91) { why = "boundaries_z=C91 is not supported by the fused stage"; return 1; }
+  if (p.thermal_conductivity_active && p.thermal_conductivity_mode != FV3D_TCM_CONSTANT) {
+    why = "conductivity_mode=B02 is not supported";
+    return 1;
+  }
+  if (p.riemann_solver != FV3D_HLL && p.riemann_solver != FV3D_HLLC) { why = "riemann_solver"; return 1; }
+  if (p.reconstruction < FV3D_PCM || p.reconstruction > FV3D_PLM) { why = "reconstruction"; return 1; }
+  if (!(p.dx > 0 && p.dy > 0 && p.dz > 0)) { why = "non-positive cell size"; return 1; }
+  return 0;
+}
+
+inline DevParams make_devparams(const fv3d_params& p, int nzc_override = 0) {
+  DevParams d;
+  std::memset(&d, 0, sizeof d);
+  d.Nx = p.Nx; d.Ny = p.Ny; d.Nz = p.Nz;
+  d.sy = p.Nx;
+  d.sz = (long long)p.Nx * p.Ny;
+  d.sf = (long long)p.Nx * p.Ny * p.Nz;
+  d.bcx = p.boundary_x; d.bcy = p.boundary_y; d.bcz = p.boundary_z;
+  d.pcm_wb = p.reconstruction == FV3D_PCM_WB;
+  d.wb_flux_zbc = p.well_balanced_flux_at_z_bc != 0;
+  d.c91_flux_zbc = p.boundary_z == FV3D_BC_C91;
+  d.gravity = p.gravity != 0;
+  d.gamma = p.gamma0;
+  d.gm1 = p.gamma0 - 1.0;
+  d.entho = 1.0 / (p.gamma0 - 1.0);
+  d.dx = p.dx; d.dy = p.dy; d.dz = p.dz;
+  d.idx = 1.0 / p.dx; d.idy = 1.0 / p.dy; d.idz = 1.0 / p.dz;
+  d.pow2x = is_pow2(p.dx); d.pow2y = is_pow2(p.dy); d.pow2z = is_pow2(p.dz);
+  d.g = p.g;
+  d.kface = 0.5 * (p.kappa + p.kappa);
+  d.bctc_zmin = p.bctc_zmin; d.bctc_zmax = p.bctc_zmax;
+  d.bctc_zmin_value = p.bctc_zmin_value; d.bctc_zmax_value = p.bctc_zmax_value;
+  d.mu = p.mu;
+  d.CFL = p.CFL;
+  d.inv_tc = inv_dt_par(p.thermal_conductivity_active != 0, p.kappa, p);
+  d.inv_visc = inv_dt_par(p.viscosity_active != 0, p.mu, p);
+  d.tiles_x = (p.Nx + STAGE_WX - 1) / STAGE_WX;
+  d.tiles_y = (p.Ny + STAGE_TY - 1) / STAGE_TY;
+  d.nzc = nzc_override > 0 ? std::max(1, std::min(nzc_override, p.Nz)) : tiles_z_for(p, d.tiles_x * d.tiles_y);
+  d.tiles_z = (p.Nz + d.nzc - 1) / d.nzc;
+  return d;
+}
+
+// ---- host layout helpers --------------------------------------------------
+// host AoS (k, j, i, var) with or without ghosts  <->  SoA domain [5][Nz][Ny][Nx]
+inline void aos_to_soa_domain(const fv3d_params& p, const double* src, int with_ghosts,
+                              std::vector<double>& dst) {
+  const long long n = (long long)p.Nx * p.Ny * p.Nz;
+  dst.resize(5 * n);
+  const int g = with_ghosts ? p.Ng : 0;
+  const long long tx = p.Nx + 2 * g, ty = p.Ny + 2 * g;
+  for (int k = 0; k < p.Nz; ++k)
+    for (int j = 0; j < p.Ny; ++j)
+      for (int i = 0; i < p.Nx; ++i) {
+        const double* s = src + (((long long)(k + g) * ty + (j + g)) * tx + (i + g)) * 5;
+        const long long d = i + (long long)p.Nx * (j + (long long)p.Ny * k);
+        for (int f = 0; f < 5; ++f) dst[f * n + d] = s[f];
+      }
+}
+
+inline void soa_domain_to_aos(const fv3d_params& p, const double* src, int with_ghosts,
+                              double* dst) {
+  const long long n = (long long)p.Nx * p.Ny * p.Nz;
+  const int g = with_ghosts ? p.Ng : 0;
+  const long long tx = p.Nx + 2 * g, ty = p.Ny + 2 * g;
+  for (int k = 0; k < p.Nz; ++k)
+    for (int j = 0; j < p.Ny; ++j)
+      for (int i = 0; i < p.Nx; ++i) {
+        double* d = dst + (((long long)(k + g) * ty + (j + g)) * tx + (i + g)) * 5;
+        const long long s = i + (long long)p.Nx * (j + (long long)p.Ny * k);
+        for (int f = 0; f < 5; ++f) d[f] = src[f * n + s];
+      }
+}
+
+// ghosted SoA [5][Ntz][Nty][Ntx]  <->  ghosted AoS (Ntz, Nty, Ntx, 5)
+inline void soa_ghosted_to_aos(const fv3d_params& p, const double* src, double* dst) {
+  const long long nt = (long long)p.Ntx * p.Nty * p.Ntz;
+  for (long long c = 0; c < nt; ++c)
+    for (int f = 0; f < 5; ++f) dst[c * 5 + f] = src[f * nt + c];
+}
+inline void aos_to_soa_ghosted(const fv3d_params& p, const double* src, double* dst) {
+  const long long nt = (long long)p.Ntx * p.Nty * p.Ntz;
+  for (long long c = 0; c < nt; ++c)
+    for (int f = 0; f < 5; ++f) dst[f * nt + c] = src[c * 5 + f];
+}
+
+}  // namespace fv3d_hip

@@ -30,6 +30,38 @@ struct StageLaunch {
 template <typename real>
 hipError_t launch_stage(const StageLaunch<real>& L, hipStream_t s);
 
+// ---- many boxes: one persistent launch per stage (fv3d_batch_impl.h) ----
+struct BoxDesc {
+  DevParams P;
+  long long off;             // element offset of the box in the pooled state buffers
+  unsigned long long* ring;  // this box's inv_dt_hyp slots (3 used)
+};
+
+struct BatchBufs {
+  void* A;     // state U^n (RK2 plain: updated in place by stage 2)
+  void* B;     // U* (plain RK2), K (compensated RK2), or the other Euler state
+  void* C;     // compensated RK2: the other U buffer
+  void* comp;  // compensated RK2: Kahan compensation
+  int* health;
+};
+
+struct BatchLaunch {
+  const BoxDesc* boxes;
+  const int2* items;      // (box, tile) work items, grouped by queue
+  const int* qstart;      // 9 offsets: queue q owns items [qstart[q], qstart[q+1])
+  unsigned* counters;     // 8 queue cursors of this stage
+  const long long* step;  // device step counter
+  BatchBufs bufs;
+  int stage, ts, last_stage, grid;
+  int recon, riemann;
+  bool tc, visc;
+  int mode;  // STAGE_PLAIN / STAGE_INC1 / STAGE_INC2
+};
+
+template <typename real>
+hipError_t launch_batch_stage(const BatchLaunch& L, hipStream_t s);
+hipError_t launch_batch_advance(long long* step, unsigned* counters, int n, hipStream_t s);
+
 // cons -> prim over the domain of U (or of U + K when K is given), written
 // into a ghosted SoA array [5][Ntz][Nty][Ntx] (ghosts untouched)
 template <typename real>

@@ -119,12 +119,17 @@ __device__ __forceinline__ void face_flux4(const Prim<real>& a, const Prim<real>
   unrotate_flux<real, DIR>(F);
 }
 
+// One tile of one box: the z chunk `tile` (x fastest, then y, then z chunk)
+// marched by the whole workgroup.  Called by the single-box kernel (one tile
+// per workgroup) and by the persistent multi-box kernel (a work list of
+// (box, tile) items).
 template <typename real, int TY, int RECON, int RS, bool TC, bool VISC, int MODE>
-__global__ void __launch_bounds__(64 * (TY + 4))
-stage_kernel(const real* __restrict__ Uin, const real* Urk, real* Uout,
-             const real* __restrict__ Kin, real* Cio,
-             const unsigned long long* __restrict__ inv_cur, unsigned long long* inv_next,
-             unsigned long long* inv_zero, int* health, DevParams P) {
+__device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const real* Urk,
+                                           real* Uout, const real* __restrict__ Kin, real* Cio,
+                                           const unsigned long long* __restrict__ inv_cur,
+                                           unsigned long long* inv_next,
+                                           unsigned long long* inv_zero, int* health,
+                                           const DevParams& P, const int tile) {
   constexpr int NR = TY + 4;
   // The primitive plane (and the temperature plane) are double-buffered by
   // plane parity unless viscosity is on, so the next plane can be published
@@ -142,12 +147,6 @@ stage_kernel(const real* __restrict__ Uin, const real* Urk, real* Uout,
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
 
-  // Blocks are dealt round-robin to the 8 XCDs (block b -> XCD b%8).  Give
-  // each XCD a contiguous run of tiles so neighbouring tiles share halo rows
-  // in that XCD's L2.
-  const int nb = gridDim.x, b = blockIdx.x;
-  const int q8 = nb >> 3, r8 = nb & 7, xcd = b & 7, loc = b >> 3;
-  const int tile = xcd * q8 + (xcd < r8 ? xcd : r8) + loc;
   const int tx = tile % P.tiles_x;
   const int t2 = tile / P.tiles_x;
   const int ty = t2 % P.tiles_y;
@@ -180,7 +179,7 @@ stage_kernel(const real* __restrict__ Uin, const real* Urk, real* Uout,
   const real gm1 = (real)P.gm1;
   const double dtd = dt_from_inv(u64_as_double(*inv_cur), P);
   const real dt = (real)dtd;
-  if (inv_zero && b == 0 && threadIdx.x == 0) *inv_zero = 0ull;
+  if (inv_zero && tile == 0 && threadIdx.x == 0) *inv_zero = 0ull;
 
   auto load_plane = [&](int kk) -> Prim<real> {
     bool fz;
@@ -567,6 +566,22 @@ stage_kernel(const real* __restrict__ Uin, const real* Urk, real* Uout,
     }
     if (bad) atomicOr(health, 1);
   }
+}
+
+template <typename real, int TY, int RECON, int RS, bool TC, bool VISC, int MODE>
+__global__ void __launch_bounds__(64 * (TY + 4))
+stage_kernel(const real* __restrict__ Uin, const real* Urk, real* Uout,
+             const real* __restrict__ Kin, real* Cio,
+             const unsigned long long* __restrict__ inv_cur, unsigned long long* inv_next,
+             unsigned long long* inv_zero, int* health, DevParams P) {
+  // Blocks are dealt round-robin to the 8 XCDs (block b -> XCD b%8).  Give
+  // each XCD a contiguous run of tiles so neighbouring tiles share halo rows
+  // in that XCD's L2.
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int q8 = nb >> 3, r8 = nb & 7, xcd = b & 7, loc = b >> 3;
+  const int tile = xcd * q8 + (xcd < r8 ? xcd : r8) + loc;
+  stage_tile<real, TY, RECON, RS, TC, VISC, MODE>(Uin, Urk, Uout, Kin, Cio, inv_cur, inv_next,
+                                                  inv_zero, health, P, tile);
 }
 
 // ---------------------------------------------------------------------------

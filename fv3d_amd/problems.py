@@ -15,6 +15,8 @@ from __future__ import annotations
 
 import numpy as np
 
+BC_REFLECTING, BC_PERIODIC = 1, 2
+
 PROBLEMS = {"sod_x": 0, "sod_y": 1, "sod_z": 2, "blast": 3, "rayleigh-taylor": 4,
             "diffusion": 5, "H84": 6, "C91": 7}
 
@@ -153,6 +155,63 @@ def polytrope_convection(p, m=1.0, amplitude=1.0e-3):
     for s, slot in enumerate((1, 2, 3)):
         d[..., slot] = amplitude * cell_uniform(p.seed, cells, -0.5, 0.5, stream=s + 1)
     return Q
+
+
+def _box_uniform(seed: int, box: int, q: int) -> float:
+    """U[0, 1) keyed by (seed, box, draw q) -- per-box parameters of an ensemble."""
+    return float(cell_uniform(seed, np.array([box * 64 + q]), 0.0, 1.0, stream=9)[0])
+
+
+def rt_box_params(base, box: int, seed: int = 12345, sides=(32, 48, 64, 96)):
+    """Parameters of box `box` of a Rayleigh-Taylor ensemble: each side drawn
+    from `sides`, gamma ~ U[1.3, 1.7], on [-1/2, 1/2]^3 (cells need not be
+    cubic), periodic x and y, reflecting z, gravity g = -0.1 (the reference's
+    sign convention: the w source is +rho*g, Update.h:149-151).  `base`
+    supplies the numerics (reconstruction, solver, time stepping, CFL)."""
+    n = [sides[min(int(_box_uniform(seed, box, a) * len(sides)), len(sides) - 1)] for a in range(3)]
+    gamma = 1.3 + 0.4 * _box_uniform(seed, box, 3)
+    return base.replace(Nx=n[0], Ny=n[1], Nz=n[2], xmin=-0.5, xmax=0.5, ymin=-0.5, ymax=0.5,
+                        zmin=-0.5, zmax=0.5, gamma0=gamma, gravity=1, g=-0.1,
+                        boundary_x=BC_PERIODIC, boundary_y=BC_PERIODIC,
+                        boundary_z=BC_REFLECTING, seed=seed + box)
+
+
+def rt_box(p, box: int, seed: int = 12345, amplitude: float = 1.0e-2, modes: int = 6):
+    """Rayleigh-Taylor box: heavy fluid (rho=2) over light (rho=1) across
+    z=0, in hydrostatic balance p = 2.5 + rho*g*z (continuous at z=0; the
+    reference's own RT problem uses p = 2.5 + 0.1*g*z for both fluids,
+    Init.h:192-200).  w carries the reference's single mode
+    (Init.h:205-208) plus `modes` random Fourier modes of integer
+    wavenumbers 1..4 in x and y with random phases, under the same z
+    envelope, scaled so that |w| <= amplitude."""
+    Q = _empty(p)
+    X, Y, Z, _ = _grid(p)
+    d = _dom(Q, p)
+    rho = np.where(Z > 0.0, 2.0, 1.0)
+    d[..., 0] = rho
+    d[..., 4] = 2.5 + rho * p.g * Z
+    single = (1.0 + np.cos(4.0 * np.pi * X)) * (1.0 + np.cos(4.0 * np.pi * Y)) / 4.0
+    rnd = np.zeros_like(X)
+    wsum = 0.0
+    for m in range(modes):
+        kx = 1 + int(4 * _box_uniform(seed, box, 8 + 4 * m))
+        ky = 1 + int(4 * _box_uniform(seed, box, 9 + 4 * m))
+        ph = 2.0 * np.pi * _box_uniform(seed, box, 10 + 4 * m)
+        a = _box_uniform(seed, box, 11 + 4 * m)
+        rnd += a * np.cos(2.0 * np.pi * (kx * X + ky * Y) + ph)
+        wsum += a
+    env = np.where(np.abs(Z) < 1.0 / 3.0, (1.0 + np.cos(3.0 * np.pi * Z)) / 2.0, 0.0)
+    d[..., 3] = amplitude * env * 0.5 * (single + rnd / max(wsum, 1e-300))
+    return Q
+
+
+def rt_ensemble(base, nbox: int, seed: int = 12345, sides=(32, 48, 64, 96)):
+    """[(Params, Q)] for an ensemble of `nbox` Rayleigh-Taylor boxes."""
+    out = []
+    for b in range(nbox):
+        p = rt_box_params(base, b, seed, sides)
+        out.append((p, rt_box(p, b, seed)))
+    return out
 
 
 def initial_condition(cfg) -> np.ndarray:

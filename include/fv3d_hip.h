@@ -104,6 +104,52 @@ int fv3d_riemann(int solver, double gamma0, long n, const double* qL, const doub
  * calibrates the profiler's byte counters (nbytes read + nbytes written). */
 int fv3d_stream_copy(size_t nbytes, int reps, float* ms_per_copy);
 
+/* ---- ensembles of independent boxes (no counterpart in the reference) ----
+ *
+ * nbox boxes, each a complete fv3d run with its own fv3d_params (sizes,
+ * gamma, boundaries, gravity, coefficients, CFL) and its own dt, advanced
+ * together.  All boxes must share reconstruction, riemann_solver,
+ * time_stepping and the conduction / viscosity switches: they select the
+ * kernel.  Each box's step is the fv3d_step of a context with the same
+ * parameters and precision, so a box's result is bit-equal to that
+ * context's.  The reference runs one box per process; this replaces a loop
+ * over such runs (main.cpp:51-76 per box). */
+typedef struct fv3d_batch fv3d_batch;
+
+#define FV3D_BATCH_GRAPH 0       /* one persistent launch per RK stage, replayed from a hipGraph */
+#define FV3D_BATCH_PERSISTENT 1  /* the same launches, issued from the host each step */
+#define FV3D_BATCH_LOOP 2        /* one single-box launch per box per stage */
+
+int fv3d_batch_create(int nbox, const fv3d_params* params, int precision, int device,
+                      fv3d_batch** out);
+int fv3d_batch_destroy(fv3d_batch* b);
+const char* fv3d_batch_last_error(const fv3d_batch* b);
+/* primToCons of box `box` from a host primitive array (layout as for
+ * fv3d_set_prim) and its first inv_dt_hyp; the other boxes are untouched. */
+int fv3d_batch_set_prim(fv3d_batch* b, int box, const double* Q, int with_ghosts);
+/* consToPrim of box `box`'s current state, ghosts filled if with_ghosts. */
+int fv3d_batch_get_prim(fv3d_batch* b, int box, double* Q, int with_ghosts);
+/* dts[nbox]: each box's CFL / max(inv_dt_hyp, inv_dt_par_tc, inv_dt_par_visc)
+ * of its current state (ComputeDt.h:17-63).  Synchronises. */
+int fv3d_batch_compute_dt(fv3d_batch* b, double* dts);
+/* nsteps steps of every box, each with its own dt; mode is FV3D_BATCH_*. */
+int fv3d_batch_step(fv3d_batch* b, int nsteps, int mode);
+/* nsteps steps timed with HIP events around the window (*ms). */
+int fv3d_batch_time(fv3d_batch* b, int nsteps, int mode, float* ms);
+long fv3d_batch_step_count(const fv3d_batch* b);
+/* total interior cells over all boxes */
+long long fv3d_batch_cells(const fv3d_batch* b);
+/* launch shape of the persistent kernels: z-chunk cap per work item (0:
+ * min(Nz, 32)) and workgroups per launch (0: one per CU). */
+int fv3d_batch_configure(fv3d_batch* b, int nzc_cap, int grid);
+/* Autotuner: times nsteps graph steps for each z-chunk cap in {8,16,32,64}
+ * and grid in {1,2,4} x CUs on a copy of the current state (restored
+ * afterwards), then configures the fastest.  cache_path (may be NULL) is a
+ * text file of "signature nzc grid" lines: a hit skips the search, a miss
+ * appends the result. */
+int fv3d_batch_tune(fv3d_batch* b, int nsteps, const char* cache_path, int* nzc_out,
+                    int* grid_out);
+
 #ifdef __cplusplus
 }
 #endif

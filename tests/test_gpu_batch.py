@@ -1,0 +1,171 @@
+"""Ensembles of independent boxes (fv3d_batch_*): every box of a batched run
+must be bit-equal to a single-box context with the same parameters and
+precision, in each of the three step modes (persistent kernel replayed from
+a hipGraph, persistent kernel issued from the host, per-box launch loop);
+FP64 boxes are additionally checked against the oracle, and FP32 ensembles
+against the FP64 ensemble in L1."""
+import numpy as np
+import pytest
+
+import fv3d_amd as F
+from fv3d_amd import problems
+from oracle.oracle import Oracle
+from tests.helpers import dom
+
+SMALL_SIDES = (16, 24, 32, 40)
+
+
+def rt_base(**kw):
+    base = dict(reconstruction=F.PLM, riemann_solver=F.HLLC, time_stepping=F.TS_RK2, CFL=0.8)
+    base.update(kw)
+    return F.Params.defaults().replace(**base)
+
+
+def single_run(p, Q0, precision, nsteps):
+    with F.Simulation(p, precision) as sim:
+        sim.set_prim(Q0)
+        dts = []
+        for _ in range(nsteps):
+            dts.append(sim.compute_dt()[0])
+            sim.step()
+        return sim.get_prim(with_ghosts=True), dts
+
+
+def batch_run(ens, precision, nsteps, mode, configure=None):
+    with F.Batch([p for p, _ in ens], precision) as b:
+        if configure:
+            b.configure(*configure)
+        for i, (_, Q0) in enumerate(ens):
+            b.set_prim(i, Q0)
+        dts = []
+        for _ in range(nsteps):
+            dts.append(b.compute_dt())
+            b.step(1, mode)
+        assert b.step_count == nsteps
+        return [b.get_prim(i, with_ghosts=True) for i in range(len(ens))], np.array(dts)
+
+
+def test_batch_rejects_mixed_numerics():
+    """No GPU needed: parameter checks come before the device is touched."""
+    a = rt_base()
+    b = rt_base(riemann_solver=F.HLL)
+    with pytest.raises(F.FV3DError) as e:
+        F.Batch([a, b], F.FP32)
+    assert e.value.code == 1
+    with pytest.raises(F.FV3DError) as e:
+        F.Batch([a, a.replace(boundary_z=F.BC_C91)], F.FP32)
+    assert e.value.code == 1
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [F.FP64, F.FP32, F.FP32_PLAIN])
+@pytest.mark.parametrize("mode", [F.BATCH_GRAPH, F.BATCH_PERSISTENT, F.BATCH_LOOP])
+def test_batch_equals_single_contexts(gpu_available, precision, mode):
+    print(f"batch vs single: precision={precision} mode={mode}", flush=True)
+    ens = problems.rt_ensemble(rt_base(), 6, seed=7, sides=SMALL_SIDES)
+    nsteps = 6
+    Qb, dtb = batch_run(ens, precision, nsteps, mode)
+    for i, (p, Q0) in enumerate(ens):
+        Qs, dts = single_run(p, Q0, precision, nsteps)
+        assert np.array_equal(dtb[:, i], np.array(dts)), i
+        assert np.array_equal(dom(p, Qb[i]), dom(p, Qs)), (i, p.Nx, p.Ny, p.Nz)
+        assert np.array_equal(Qb[i], Qs), i  # ghost shell from the same fill
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("configure", [(8, 0), (64, 1024), (1, 300)])
+def test_batch_launch_shape_does_not_change_results(gpu_available, configure):
+    print(f"batch launch shape {configure}", flush=True)
+    ens = problems.rt_ensemble(rt_base(), 5, seed=11, sides=SMALL_SIDES)
+    Qa, da = batch_run(ens, F.FP32, 4, F.BATCH_GRAPH)
+    Qb, db = batch_run(ens, F.FP32, 4, F.BATCH_GRAPH, configure=configure)
+    assert np.array_equal(da, db)
+    for a, b in zip(Qa, Qb):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.gpu
+def test_batch_fp64_boxes_match_oracle(gpu_available):
+    ens = problems.rt_ensemble(rt_base(), 3, seed=3, sides=(16, 24))
+    nsteps = 5
+    Qb, dtb = batch_run(ens, F.FP64, nsteps, F.BATCH_GRAPH)
+    for i, (p, Q0) in enumerate(ens):
+        o = Oracle(p)
+        U, _, dts, _ = o.run(Q0, nsteps, stop_at_tend=False)
+        assert np.array_equal(dtb[:, i], dts), i
+        Qo = o.cons_to_prim(U)
+        assert np.array_equal(dom(p, Qb[i]), dom(p, Qo)), i
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ts", [F.TS_EULER, F.TS_RK2])
+def test_batch_sweep_kappa_mu_gamma(gpu_available, ts):
+    """Per-box conduction and viscosity coefficients and gamma (a parameter
+    sweep), with fixed-temperature walls; Euler and RK2."""
+    print(f"batch sweep ts={ts}", flush=True)
+    base = rt_base(time_stepping=ts, riemann_solver=F.HLL, thermal_conductivity_active=1,
+                   viscosity_active=1, CFL=0.3, bctc_zmin=F.BCTC_FIXED_TEMPERATURE,
+                   bctc_zmax=F.BCTC_FIXED_TEMPERATURE, bctc_zmin_value=2.5, bctc_zmax_value=1.2)
+    ens = []
+    for b, (kappa, mu) in enumerate([(0.0, 0.0), (1e-3, 2e-3), (5e-3, 1e-3), (2e-3, 8e-3)]):
+        p = problems.rt_box_params(base, b, seed=5, sides=(16, 24)).replace(kappa=kappa, mu=mu)
+        ens.append((p, problems.rt_box(p, b, seed=5)))
+    nsteps = 5
+    for precision in (F.FP64, F.FP32):
+        Qb, dtb = batch_run(ens, precision, nsteps, F.BATCH_GRAPH)
+        for i, (p, Q0) in enumerate(ens):
+            Qs, dts = single_run(p, Q0, precision, nsteps)
+            assert np.array_equal(dtb[:, i], np.array(dts)), (precision, i)
+            assert np.array_equal(Qb[i], Qs), (precision, i)
+
+
+@pytest.mark.gpu
+def test_batch_fp32_ensemble_l1_against_fp64(gpu_available):
+    """L1(rho) of the FP32 (Kahan RK2) ensemble against the FP64 ensemble,
+    per box, after 100 steps: <= 1e-5."""
+    ens = problems.rt_ensemble(rt_base(), 8, seed=12345, sides=(32, 48))
+    nsteps = 100
+    ps = [p for p, _ in ens]
+    out = {}
+    for precision in (F.FP64, F.FP32):
+        with F.Batch(ps, precision) as b:
+            for i, (_, Q0) in enumerate(ens):
+                b.set_prim(i, Q0)
+            b.step(nsteps, F.BATCH_GRAPH)
+            out[precision] = [b.get_prim(i, with_ghosts=False) for i in range(len(ens))]
+    for i in range(len(ens)):
+        l1 = np.mean(np.abs(out[F.FP32][i][..., 0] - out[F.FP64][i][..., 0]))
+        print(f"box {i}: L1(rho) = {l1:.3e}")
+        assert l1 <= 1e-5, (i, l1)
+
+
+@pytest.mark.gpu
+def test_batch_tune_restores_state(gpu_available, tmp_path):
+    ens = problems.rt_ensemble(rt_base(), 4, seed=9, sides=SMALL_SIDES)
+    ps = [p for p, _ in ens]
+    cache = str(tmp_path / "tune.txt")
+    with F.Batch(ps, F.FP32) as b:
+        for i, (_, Q0) in enumerate(ens):
+            b.set_prim(i, Q0)
+        b.step(2)
+        before = [b.get_prim(i) for i in range(len(ens))]
+        dt0 = b.compute_dt()
+        nzc, grid = b.tune(nsteps=2, cache=cache)
+        assert nzc in (8, 16, 32, 64) and grid > 0
+        assert b.step_count == 2
+        assert np.array_equal(b.compute_dt(), dt0)
+        for i in range(len(ens)):
+            assert np.array_equal(b.get_prim(i), before[i])
+        b.step(3)
+        after = [b.get_prim(i) for i in range(len(ens))]
+    # a cache hit configures the same shape without a search, and the run
+    # continues identically
+    with F.Batch(ps, F.FP32) as b:
+        for i, (_, Q0) in enumerate(ens):
+            b.set_prim(i, Q0)
+        b.step(2)
+        assert b.tune(nsteps=2, cache=cache) == (nzc, grid)
+        b.step(3)
+        for i in range(len(ens)):
+            assert np.array_equal(b.get_prim(i), after[i])
+    assert open(cache).read().count("\n") == 1
