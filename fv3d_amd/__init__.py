@@ -166,6 +166,9 @@ def lib() -> ctypes.CDLL:
         "fv3d_fill_boundaries": (ctypes.c_int, [P(Params), dp]),
         "fv3d_riemann": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_long, dp, dp, dp, dp]),
         "fv3d_stream_copy": (ctypes.c_int, [ctypes.c_size_t, ctypes.c_int, P(ctypes.c_float)]),
+        "fv3d_forward_frozen": (ctypes.c_int, [vp, ctypes.c_int, dp, P(ctypes.c_float)]),
+        "fv3d_adjoint_run": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, dp, dp, dp, dp, dp]),
+        "fv3d_adjoint_times": (ctypes.c_int, [vp, P(ctypes.c_float)]),
         "fv3d_batch_create": (ctypes.c_int, [ctypes.c_int, P(Params), ctypes.c_int, ctypes.c_int, P(vp)]),
         "fv3d_batch_destroy": (ctypes.c_int, [vp]),
         "fv3d_batch_last_error": (ctypes.c_char_p, [vp]),
@@ -298,6 +301,35 @@ class Simulation:
             self._ctx, int(n), ctypes.byref(tot),
             st.ctypes.data_as(ctypes.POINTER(ctypes.c_float)) if per_stage else None), self._ctx)
         return tot.value, st
+
+    # ---- frozen-dt runs and the discrete adjoint ----
+    def forward_frozen(self, dts) -> float:
+        """Advance the state by len(dts) steps with the given dt sequence;
+        returns the device time in ms."""
+        dts = np.ascontiguousarray(dts, dtype=np.float64)
+        ms = ctypes.c_float()
+        _check(lib().fv3d_forward_frozen(self._ctx, len(dts), _dptr(dts), ctypes.byref(ms)), self._ctx)
+        return ms.value
+
+    def adjoint(self, dts, dJ_dUT: np.ndarray, K: int = 20):
+        """dJ/dU0 (Nz, Ny, Nx, 5), dJ/dkappa, dJ/dmu for J(U_T) after the
+        frozen-dt run of len(dts) steps from the current state, given
+        dJ/dU_T (Nz, Ny, Nx, 5).  Checkpoints every K steps."""
+        dts = np.ascontiguousarray(dts, dtype=np.float64)
+        lamT = np.ascontiguousarray(dJ_dUT, dtype=np.float64)
+        if lamT.shape != self.params.shape_domain:
+            raise ValueError(f"dJ_dUT has shape {lamT.shape}, expected {self.params.shape_domain}")
+        out = np.zeros(self.params.shape_domain)
+        dk, dm = ctypes.c_double(), ctypes.c_double()
+        _check(lib().fv3d_adjoint_run(self._ctx, len(dts), int(K), _dptr(dts), _dptr(lamT), _dptr(out),
+                                      ctypes.byref(dk), ctypes.byref(dm)), self._ctx)
+        return out, dk.value, dm.value
+
+    def adjoint_times(self):
+        """(forward pass, backward pass, total) device ms of the last adjoint run."""
+        ms = (ctypes.c_float * 3)()
+        _check(lib().fv3d_adjoint_times(self._ctx, ms), self._ctx)
+        return tuple(ms)
 
 
 BATCH_GRAPH, BATCH_PERSISTENT, BATCH_LOOP = 0, 1, 2

@@ -30,11 +30,13 @@ HIP_FLAGS = [
 ]
 
 SOURCES = ["fv3d_stage.hip", "fv3d_stage_f32.hip", "fv3d_batch.hip", "fv3d_batch_f32.hip",
-           "fv3d_aux.hip", "fv3d_api.hip", "fv3d_batch_api.hip"]
+           "fv3d_aux.hip", "fv3d_api.hip", "fv3d_batch_api.hip", "fv3d_adjoint.hip",
+           "fv3d_adjoint_f32.hip", "fv3d_adjoint_api.hip"]
 # per-source extra flags
 EXTRA = {
     "fv3d_stage_f32.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt"],
     "fv3d_batch_f32.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt"],
+    "fv3d_adjoint_f32.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt"],
 }
 
 

@@ -1,0 +1,699 @@
+// fv3d_adjoint_impl.h -- discrete adjoint of one stage of the fv3d update.
+//
+// A stage is S(U) = U + dt R(prim(U)) with dt frozen (taken from the
+// forward run), R the flux divergence of the reconstructed Riemann fluxes
+// (Update.h:90-162) plus conduction (ThermalConduction.h:24-98) and
+// viscosity (Viscosity.h:14-155).  Given the adjoint v of the stage output
+// these kernels form
+//     out = sigma*v + J_S(U)^T (sigma*v) - sigma*v + alpha*extra
+//         = sigma*v + dt J_R(U)^T (sigma*v) + alpha*extra
+// in three steps:
+//   k_adj_faces      one thread per face instance: recomputes the face
+//                    (slopes with their minmod branches, face states, the
+//                    Riemann branch) and scatters the vector-Jacobian
+//                    product into a primitive-space accumulator Qb
+//                    [6][N] = (rho, u, v, w, p, T) at the four stencil cells,
+//                    through the ghost maps (reflected ghosts negate the
+//                    normal velocity, so does their adjoint);
+//   k_adj_diffusion  one thread per cell: the transposes of the conduction
+//                    (into the T slot) and viscosity stencils, including the
+//                    reference's viscosity index quirks, and the parameter
+//                    gradients dS/dkappa, dS/dmu dotted with sigma*v;
+//   k_adj_finalize   one thread per cell: Qb -> conserved through the
+//                    transpose of consToPrim (T = p/rho first), plus the
+//                    identity and the RK2 weights; clears Qb.
+// Non-smooth points keep the branch the forward evaluation takes (minmod
+// selection, HLL/HLLC wave region, fmin/fmax operands).  The density floor
+// (Update.h:160) is taken as inactive.  Accumulation into Qb uses atomics,
+// so sums may associate differently from run to run.
+#pragma once
+
+#include "fv3d_device.h"
+#include "fv3d_kernels.h"
+
+namespace fv3d_hip {
+
+// ---------------------------------------------------------------------------
+// reverse Riemann solvers (face frame, arrays r,u,v,w,p); Lb/Rb accumulate
+// ---------------------------------------------------------------------------
+template <typename real>
+__device__ __forceinline__ void hllc_vjp(const real L[5], const real R[5], real gamma, real entho,
+                                         const real Fb[5], real Lb[5], real Rb[5]) {
+  const real rL = L[0], uL = L[1], vL = L[2], pL = L[4];
+  const real rR = R[0], uR = R[1], vR = R[2], pR = R[4];
+  const real h = (real)0.5;
+  const real EL = h * rL * (uL * uL + vL * vL) + pL * entho;
+  const real ER = h * rR * (uR * uR + vR * vR) + pR * entho;
+  const real cL = sqrt(pL * gamma / rL), cR = sqrt(pR * gamma / rR);
+  const real cmax = fmax(cL, cR);
+  const real SL = fmin(uL, uR) - cmax, SR = fmax(uL, uR) + cmax;
+  const real rcL = rL * (uL - SL), rcR = rR * (SR - uR);
+  const real den = rcR + rcL;
+  const real uS = (rcR * uR + rcL * uL + (pL - pR)) / den;
+  const real pS = (rcR * pL + rcL * pR + rcL * rcR * (uL - uR)) / den;
+  const bool kl = uS > (real)0;
+  const real rK = kl ? rL : rR, uK = kl ? uL : uR, pK = kl ? pL : pR, EK = kl ? EL : ER,
+             SK = kl ? SL : SR;
+  const real A = SK - uK, D = SK - uS;
+  const real rS = rK * A / D;
+  const real ES = (A * EK - pK * uK + pS * uS) / D;
+  const bool left = SL > (real)0;
+  const bool star = !left && (kl || SR > (real)0);
+  const real sr = star ? rS : (left ? rL : rR);
+  const real su = star ? uS : (left ? uL : uR);
+  const real sp = star ? pS : (left ? pL : pR);
+  const real E = star ? ES : (left ? EL : ER);
+  const bool vw_left = star ? kl : left;
+  const real sv = vw_left ? vL : R[2];
+  const real sw = vw_left ? L[3] : R[3];
+  const real F0 = sr * su;
+
+  // F = (sr su, sr su^2 + sp, F0 sv, F0 sw, (E + sp) su)
+  const real tv = Fb[2] * sv + Fb[3] * sw;
+  const real srb = Fb[0] * su + Fb[1] * su * su + tv * su;
+  const real sub = Fb[0] * sr + Fb[1] * (real)2 * sr * su + tv * sr + Fb[4] * (E + sp);
+  const real spb = Fb[1] + Fb[4] * su;
+  const real Eb = Fb[4] * su;
+  if (vw_left) { Lb[2] += Fb[2] * F0; Lb[3] += Fb[3] * F0; }
+  else { Rb[2] += Fb[2] * F0; Rb[3] += Fb[3] * F0; }
+
+  real ELb = 0, ERb = 0, SLb = 0, SRb = 0, uSb = 0, pSb = 0;
+  if (star) {
+    uSb += sub;
+    pSb += spb;
+    real Ab = srb * rK / D, Db = -srb * rS / D;
+    const real rKb = srb * A / D;
+    const real NEb = Eb / D;
+    Db -= Eb * ES / D;
+    Ab += NEb * EK;
+    const real EKb = NEb * A;
+    const real pKb = -NEb * uK;
+    real uKb = -NEb * pK;
+    pSb += NEb * uS;
+    uSb += NEb * pS;
+    const real SKb = Ab + Db;
+    uKb -= Ab;
+    uSb -= Db;
+    if (kl) { Lb[0] += rKb; Lb[1] += uKb; Lb[4] += pKb; ELb += EKb; SLb += SKb; }
+    else { Rb[0] += rKb; Rb[1] += uKb; Rb[4] += pKb; ERb += EKb; SRb += SKb; }
+  } else if (left) {
+    Lb[0] += srb; Lb[1] += sub; Lb[4] += spb; ELb += Eb;
+  } else {
+    Rb[0] += srb; Rb[1] += sub; Rb[4] += spb; ERb += Eb;
+  }
+  const real NUb = uSb / den, NPb = pSb / den;
+  const real denb = -(uSb * uS + pSb * pS) / den;
+  real rcRb = NUb * uR + denb, rcLb = NUb * uL + denb;
+  Rb[1] += NUb * rcR;
+  Lb[1] += NUb * rcL;
+  Lb[4] += NUb;
+  Rb[4] -= NUb;
+  const real du = uL - uR;
+  rcRb += NPb * (pL + rcL * du);
+  rcLb += NPb * (pR + rcR * du);
+  Lb[4] += NPb * rcR;
+  Rb[4] += NPb * rcL;
+  Lb[1] += NPb * rcL * rcR;
+  Rb[1] -= NPb * rcL * rcR;
+  Lb[0] += rcLb * (uL - SL);
+  Lb[1] += rcLb * rL;
+  SLb -= rcLb * rL;
+  Rb[0] += rcRb * (SR - uR);
+  SRb += rcRb * rR;
+  Rb[1] -= rcRb * rR;
+  const real cmaxb = SRb - SLb;
+  if (uL < uR) Lb[1] += SLb; else Rb[1] += SLb;
+  if (uL > uR) Lb[1] += SRb; else Rb[1] += SRb;
+  const real cLb = cL >= cR ? cmaxb : (real)0, cRb = cL >= cR ? (real)0 : cmaxb;
+  Lb[4] += cLb * gamma / ((real)2 * rL * cL);
+  Lb[0] -= cLb * cL / ((real)2 * rL);
+  Rb[4] += cRb * gamma / ((real)2 * rR * cR);
+  Rb[0] -= cRb * cR / ((real)2 * rR);
+  // E = 0.5 r (u^2 + v^2) + p entho: the second transverse velocity is
+  // dropped (RiemannSolvers.h:52-55), so it has no adjoint here
+  Lb[0] += ELb * h * (uL * uL + vL * vL); Lb[1] += ELb * rL * uL; Lb[2] += ELb * rL * vL;
+  Lb[4] += ELb * entho;
+  Rb[0] += ERb * h * (uR * uR + vR * vR); Rb[1] += ERb * rR * uR; Rb[2] += ERb * rR * vR;
+  Rb[4] += ERb * entho;
+}
+
+// Euler flux of a primitive state (States.h:113-127) reversed
+template <typename real>
+__device__ __forceinline__ void euler_flux_vjp(const real q[5], real gm1, const real Fb[5],
+                                               real qb[5]) {
+  const real r = q[0], u = q[1], v = q[2], w = q[3], p = q[4];
+  const real k2 = u * u + v * v + w * w;
+  const real Ef = p / gm1 + (real)0.5 * r * k2;
+  qb[0] += Fb[0] * u + Fb[1] * u * u + Fb[2] * u * v + Fb[3] * u * w + Fb[4] * u * (real)0.5 * k2;
+  qb[1] += Fb[0] * r + Fb[1] * (real)2 * r * u + Fb[2] * r * v + Fb[3] * r * w +
+           Fb[4] * ((p + Ef) + u * r * u);
+  qb[2] += Fb[2] * r * u + Fb[4] * u * r * v;
+  qb[3] += Fb[3] * r * u + Fb[4] * u * r * w;
+  qb[4] += Fb[1] + Fb[4] * u * ((real)1 + (real)1 / gm1);
+}
+
+// conserved variables of a primitive state (States.h:20-31) reversed
+template <typename real>
+__device__ __forceinline__ void prim_to_cons_vjp(const real q[5], real gm1, const real Ub[5],
+                                                 real qb[5]) {
+  const real r = q[0], u = q[1], v = q[2], w = q[3];
+  qb[0] += Ub[0] + Ub[1] * u + Ub[2] * v + Ub[3] * w + Ub[4] * (real)0.5 * (u * u + v * v + w * w);
+  qb[1] += Ub[1] * r + Ub[4] * r * u;
+  qb[2] += Ub[2] * r + Ub[4] * r * v;
+  qb[3] += Ub[3] * r + Ub[4] * r * w;
+  qb[4] += Ub[4] / gm1;
+}
+
+template <typename real>
+__device__ __forceinline__ void hll_vjp(const real L[5], const real R[5], real gamma, real gm1,
+                                        const real Fb[5], real Lb[5], real Rb[5]) {
+  const real aL = sqrt(L[4] * gamma / L[0]), aR = sqrt(R[4] * gamma / R[0]);
+  const real xL = L[1] - aL, xR = R[1] - aR, yL = L[1] + aL, yR = R[1] + aR;
+  const real SL = fmin(xL, xR), SR = fmax(yL, yR);
+  if (SL >= (real)0) { euler_flux_vjp(L, gm1, Fb, Lb); return; }
+  if (SR <= (real)0) { euler_flux_vjp(R, gm1, Fb, Rb); return; }
+  real FL[5], FR[5], UL[5], UR[5];
+  auto flux = [&](const real q[5], real F[5], real U[5]) {
+    const real r = q[0], u = q[1], v = q[2], w = q[3], p = q[4];
+    const real Ef = p / gm1 + (real)0.5 * r * (u * u + v * v + w * w);
+    F[0] = r * u; F[1] = r * u * u + p; F[2] = r * u * v; F[3] = r * u * w; F[4] = (p + Ef) * u;
+    U[0] = r; U[1] = r * u; U[2] = r * v; U[3] = r * w; U[4] = Ef;
+  };
+  flux(L, FL, UL);
+  flux(R, FR, UR);
+  const real dS = SR - SL;
+  real FLb[5], FRb[5], ULb[5], URb[5];
+  real SLb = 0, SRb = 0;
+#pragma unroll
+  for (int n = 0; n < 5; ++n) {
+    const real dU = UR[n] - UL[n];
+    const real F = (FL[n] * SR - FR[n] * SL + dU * SL * SR) / dS;
+    FLb[n] = Fb[n] * SR / dS;
+    FRb[n] = -Fb[n] * SL / dS;
+    URb[n] = Fb[n] * SL * SR / dS;
+    ULb[n] = -URb[n];
+    SRb += Fb[n] * (FL[n] + dU * SL - F) / dS;
+    SLb += Fb[n] * (-FR[n] + dU * SR + F) / dS;
+  }
+  euler_flux_vjp(L, gm1, FLb, Lb);
+  euler_flux_vjp(R, gm1, FRb, Rb);
+  prim_to_cons_vjp(L, gm1, ULb, Lb);
+  prim_to_cons_vjp(R, gm1, URb, Rb);
+  real aLb = 0, aRb = 0;
+  if (xL < xR) { Lb[1] += SLb; aLb -= SLb; } else { Rb[1] += SLb; aRb -= SLb; }
+  if (yL > yR) { Lb[1] += SRb; aLb += SRb; } else { Rb[1] += SRb; aRb += SRb; }
+  Lb[4] += aLb * gamma / ((real)2 * L[0] * aL);
+  Lb[0] -= aLb * aL / ((real)2 * L[0]);
+  Rb[4] += aRb * gamma / ((real)2 * R[0] * aR);
+  Rb[0] -= aRb * aR / ((real)2 * R[0]);
+}
+
+// minmod(a, b) branch: 0 -> zero, 1 -> a, 2 -> b (same tests as minmod)
+template <typename real>
+__device__ __forceinline__ int minmod_branch(real a, real b) {
+  return (a * b < (real)0) ? 0 : ((fabs(a) < fabs(b)) ? 1 : 2);
+}
+template <typename real>
+__device__ __forceinline__ real minmod_sel(real a, real b, int br) {
+  return br == 0 ? (real)0 : (br == 1 ? a : b);
+}
+
+// Slope at b from a, b, c (x, y: minmod(back, fwd); z: minmod(fwd, back)),
+// with the selection of every variable: +1 backward, -1 forward, 0 zero.
+template <typename real, bool ZORD>
+__device__ __forceinline__ void slope_sel(const real a[5], const real b[5], const real c[5],
+                                          real s[5], int sel[5]) {
+#pragma unroll
+  for (int n = 0; n < 5; ++n) {
+    const real bk = b[n] - a[n], fw = c[n] - b[n];
+    if (!ZORD) {
+      const int br = minmod_branch(bk, fw);
+      s[n] = minmod_sel(bk, fw, br);
+      sel[n] = br == 0 ? 0 : (br == 1 ? 1 : -1);
+    } else {
+      const int br = minmod_branch(fw, bk);
+      s[n] = minmod_sel(fw, bk, br);
+      sel[n] = br == 0 ? 0 : (br == 1 ? -1 : 1);
+    }
+  }
+}
+template <typename real>
+__device__ __forceinline__ void slope_vjp(const int sel[5], const real sb[5], real ab[5],
+                                          real bb[5], real cb[5]) {
+#pragma unroll
+  for (int n = 0; n < 5; ++n) {
+    if (sel[n] > 0) { bb[n] += sb[n]; ab[n] -= sb[n]; }
+    else if (sel[n] < 0) { cb[n] += sb[n]; bb[n] -= sb[n]; }
+  }
+}
+
+template <int DIR, typename real>
+__device__ __forceinline__ void swap_dir(real x[5]) {
+  if (DIR == FV3D_IY) { const real t = x[1]; x[1] = x[2]; x[2] = t; }
+  if (DIR == FV3D_IZ) { const real t = x[1]; x[1] = x[3]; x[3] = t; }
+}
+
+template <typename real>
+__device__ __forceinline__ void load_prim_arr(const real* __restrict__ U, long long cell,
+                                              long long sf, real gm1, real q[5]) {
+  const Prim<real> p = cons_to_prim(U[cell], U[cell + sf], U[cell + 2 * sf], U[cell + 3 * sf],
+                                    U[cell + 4 * sf], gm1);
+  q[0] = p.r; q[1] = p.u; q[2] = p.v; q[3] = p.w; q[4] = p.p;
+}
+
+template <typename real>
+__device__ __forceinline__ void atomic_add(real* p, real v) {
+  if (v != (real)0) unsafeAtomicAdd(p, v);
+}
+
+// ---------------------------------------------------------------------------
+// faces.  Instance m+1/2 along DIR, m in [-1, N-1], stencil m-1..m+2;
+// its flux enters cell m (if inside) with -dt/d and cell m+1 (if inside)
+// with +dt/d.
+// ---------------------------------------------------------------------------
+template <typename real, int DIR, int RECON, int RS>
+__global__ void __launch_bounds__(256)
+k_adj_faces(const real* __restrict__ U, const real* __restrict__ v, real sigma, real* Qb,
+            double dt, DevParams P) {
+  const int Nd = DIR == FV3D_IX ? P.Nx : (DIR == FV3D_IY ? P.Ny : P.Nz);
+  const long long nf = (long long)(P.Nx + (DIR == FV3D_IX)) * (P.Ny + (DIR == FV3D_IY)) *
+                       (P.Nz + (DIR == FV3D_IZ));
+  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (t >= nf) return;
+  const int ex = P.Nx + (DIR == FV3D_IX), ey = P.Ny + (DIR == FV3D_IY);
+  const int fi = (int)(t % ex), fj = (int)((t / ex) % ey), fk = (int)(t / ((long long)ex * ey));
+  const int m = (DIR == FV3D_IX ? fi : (DIR == FV3D_IY ? fj : fk)) - 1;
+  const int bc = DIR == FV3D_IX ? P.bcx : (DIR == FV3D_IY ? P.bcy : P.bcz);
+  const long long stride = DIR == FV3D_IX ? 1 : (DIR == FV3D_IY ? P.sy : P.sz);
+  const long long base = (DIR == FV3D_IX ? 0 : fi) + (DIR == FV3D_IY ? 0 : (long long)fj * P.sy) +
+                         (DIR == FV3D_IZ ? 0 : (long long)fk * P.sz);
+  const real gm1 = (real)P.gm1;
+  real q[4][5];
+  long long src[4];
+  bool flip[4];
+#pragma unroll
+  for (int o = 0; o < 4; ++o) {
+    const int s = map_coord(m - 1 + o, Nd, bc, flip[o]);
+    src[o] = base + (long long)s * stride;
+    load_prim_arr(U, src[o], P.sf, gm1, q[o]);
+    if (flip[o]) q[o][1 + DIR] = -q[o][1 + DIR];
+  }
+  // adjoint of the face flux (lab frame)
+  const double d = DIR == FV3D_IX ? P.dx : (DIR == FV3D_IY ? P.dy : P.dz);
+  const real coef = (real)((double)sigma * dt / d);
+  real Fb[5];
+#pragma unroll
+  for (int n = 0; n < 5; ++n) {
+    const real vc = (m + 1 < Nd) ? v[src[2] + n * P.sf] : (real)0;
+    const real vb = (m >= 0) ? v[src[1] + n * P.sf] : (real)0;
+    Fb[n] = coef * (vc - vb);
+  }
+  real qL[5], qR[5], sb[5], sc[5];
+  int selb[5] = {0, 0, 0, 0, 0}, selc[5] = {0, 0, 0, 0, 0};
+  if (RECON == FV3D_PLM) {
+    slope_sel<real, DIR == FV3D_IZ>(q[0], q[1], q[2], sb, selb);
+    slope_sel<real, DIR == FV3D_IZ>(q[1], q[2], q[3], sc, selc);
+#pragma unroll
+    for (int n = 0; n < 5; ++n) {
+      qL[n] = q[1][n] + sb[n] * (real)0.5;
+      qR[n] = q[2][n] - sc[n] * (real)0.5;
+    }
+  } else {
+#pragma unroll
+    for (int n = 0; n < 5; ++n) { qL[n] = q[1][n]; qR[n] = q[2][n]; }
+  }
+  swap_dir<DIR>(qL);
+  swap_dir<DIR>(qR);
+  swap_dir<DIR>(Fb);
+  real Lb[5] = {0, 0, 0, 0, 0}, Rb[5] = {0, 0, 0, 0, 0};
+  if (RS == FV3D_HLL) hll_vjp(qL, qR, (real)P.gamma, gm1, Fb, Lb, Rb);
+  else hllc_vjp(qL, qR, (real)P.gamma, (real)P.entho, Fb, Lb, Rb);
+  swap_dir<DIR>(Lb);
+  swap_dir<DIR>(Rb);
+  real qb[4][5];
+#pragma unroll
+  for (int n = 0; n < 5; ++n) {
+    qb[0][n] = 0; qb[1][n] = Lb[n]; qb[2][n] = Rb[n]; qb[3][n] = 0;
+  }
+  if (RECON == FV3D_PLM) {
+    real ssb[5], ssc[5];
+#pragma unroll
+    for (int n = 0; n < 5; ++n) { ssb[n] = (real)0.5 * Lb[n]; ssc[n] = (real)-0.5 * Rb[n]; }
+    slope_vjp(selb, ssb, qb[0], qb[1], qb[2]);
+    slope_vjp(selc, ssc, qb[1], qb[2], qb[3]);
+  }
+#pragma unroll
+  for (int o = 0; o < 4; ++o) {
+    if (flip[o]) qb[o][1 + DIR] = -qb[o][1 + DIR];
+#pragma unroll
+    for (int n = 0; n < 5; ++n) atomic_add(&Qb[src[o] + n * P.sf], qb[o][n]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// conduction and viscosity (one thread per cell)
+// ---------------------------------------------------------------------------
+
+// The viscous update of one cell, Viscosity.h:28-153 with mu = 1, as a
+// function of its 3x3x3 velocity stencil S[dk][dj][di][c] (c = u, v, w):
+// out[0..3] = increments of rho u, rho v, rho w, E (before * mu * dt).
+// In reverse mode (REV) it accumulates d(out . g)/dS into Sb instead.
+template <typename real, bool REV>
+__device__ __forceinline__ void visc_cell(const real (&S)[3][3][3][3], real oodx, real oody,
+                                          real oodz, const real g[4], real out[4],
+                                          real (&Sb)[3][3][3][3]) {
+  const real f43 = (real)(4.0 / 3.0), f23 = (real)(2.0 / 3.0), h = (real)0.5, q4 = (real)0.25;
+#pragma unroll
+  for (int n = 0; n < 4; ++n) out[n] = 0;
+#pragma unroll
+  for (int s = 1; s < 3; ++s) {
+    const real sgn = s == 1 ? (real)-1 : (real)1;
+    // ---- X: S(1,1,s) and S(1,1,s-1), y and z differences around them
+    {
+      const real qiu = h * (S[1][1][s][0] + S[1][1][s - 1][0]);
+      const real qiv = h * (S[1][1][s][1] + S[1][1][s - 1][1]);
+      const real qiw = h * (S[1][1][s][2] + S[1][1][s - 1][2]);
+      const real dudx = oodx * (S[1][1][s][0] - S[1][1][s - 1][0]);
+      const real dvdx = oodx * (S[1][1][s][1] - S[1][1][s - 1][1]);
+      const real dwdx = oodx * (S[1][1][s][2] - S[1][1][s - 1][2]);
+      const real dudy = q4 * oody * (S[1][2][s][0] - S[1][0][s][0] + S[1][2][s - 1][0] - S[1][0][s - 1][0]);
+      const real dvdy = q4 * oody * (S[1][2][s][1] - S[1][0][s][1] + S[1][2][s - 1][1] - S[1][0][s - 1][1]);
+      const real dudz = q4 * oodz * (S[2][1][s][0] - S[0][1][s][0] + S[2][1][s - 1][0] - S[0][1][s - 1][0]);
+      const real dwdz = q4 * oodz * (S[2][1][s][2] - S[0][1][s][2] + S[2][1][s - 1][2] - S[0][1][s - 1][2]);
+      const real txx = f43 * dudx - f23 * (dvdy + dwdz);
+      const real txy = dvdx + dudy;
+      const real txz = dwdx + dudz;
+      if (!REV) {
+        out[0] += sgn * txx; out[1] += sgn * txy; out[2] += sgn * txz;
+        out[3] += sgn * (txx * qiu + txy * qiv + txz * qiw);
+      } else {
+        const real txxb = sgn * (g[0] + g[3] * qiu), txyb = sgn * (g[1] + g[3] * qiv),
+                   txzb = sgn * (g[2] + g[3] * qiw);
+        const real qiub = sgn * g[3] * txx * h, qivb = sgn * g[3] * txy * h, qiwb = sgn * g[3] * txz * h;
+        Sb[1][1][s][0] += qiub; Sb[1][1][s - 1][0] += qiub;
+        Sb[1][1][s][1] += qivb; Sb[1][1][s - 1][1] += qivb;
+        Sb[1][1][s][2] += qiwb; Sb[1][1][s - 1][2] += qiwb;
+        const real dudxb = f43 * txxb * oodx, dvdyb = -f23 * txxb * q4 * oody, dwdzb = -f23 * txxb * q4 * oodz;
+        const real dvdxb = txyb * oodx, dudyb = txyb * q4 * oody;
+        const real dwdxb = txzb * oodx, dudzb = txzb * q4 * oodz;
+        Sb[1][1][s][0] += dudxb; Sb[1][1][s - 1][0] -= dudxb;
+        Sb[1][1][s][1] += dvdxb; Sb[1][1][s - 1][1] -= dvdxb;
+        Sb[1][1][s][2] += dwdxb; Sb[1][1][s - 1][2] -= dwdxb;
+        Sb[1][2][s][0] += dudyb; Sb[1][0][s][0] -= dudyb; Sb[1][2][s - 1][0] += dudyb; Sb[1][0][s - 1][0] -= dudyb;
+        Sb[1][2][s][1] += dvdyb; Sb[1][0][s][1] -= dvdyb; Sb[1][2][s - 1][1] += dvdyb; Sb[1][0][s - 1][1] -= dvdyb;
+        Sb[2][1][s][0] += dudzb; Sb[0][1][s][0] -= dudzb; Sb[2][1][s - 1][0] += dudzb; Sb[0][1][s - 1][0] -= dudzb;
+        Sb[2][1][s][2] += dwdzb; Sb[0][1][s][2] -= dwdzb; Sb[2][1][s - 1][2] += dwdzb; Sb[0][1][s - 1][2] -= dwdzb;
+      }
+    }
+    // ---- Y: S(1,s,1) and S(1,s-1,1)
+    {
+      const real qiu = h * (S[1][s][1][0] + S[1][s - 1][1][0]);
+      const real qiv = h * (S[1][s][1][1] + S[1][s - 1][1][1]);
+      const real qiw = h * (S[1][s][1][2] + S[1][s - 1][1][2]);
+      const real dudy = oody * (S[1][s][1][0] - S[1][s - 1][1][0]);
+      const real dvdy = oody * (S[1][s][1][1] - S[1][s - 1][1][1]);
+      const real dwdy = oody * (S[1][s][1][2] - S[1][s - 1][1][2]);
+      const real dudx = q4 * oodx * (S[1][s][2][0] - S[1][s][0][0] + S[1][s - 1][2][0] - S[1][s - 1][0][0]);
+      const real dvdx = q4 * oodx * (S[1][s][2][1] - S[1][s][0][1] + S[1][s - 1][2][1] - S[1][s - 1][0][1]);
+      const real dwdz = q4 * oodz * (S[2][s][1][2] - S[0][s][1][2] + S[2][s - 1][1][2] - S[0][s - 1][1][2]);
+      const real dvdz = q4 * oodz * (S[2][s][1][1] - S[0][s][1][1] + S[2][s - 1][1][1] - S[0][s - 1][1][1]);
+      const real tyy = f43 * dvdy - f23 * (dudx + dwdz);
+      const real txy = dvdx + dudy;
+      const real tyz = dvdz + dwdy;
+      if (!REV) {
+        out[0] += sgn * txy; out[1] += sgn * tyy; out[2] += sgn * tyz;
+        out[3] += sgn * (txy * qiu + tyy * qiv + tyz * qiw);
+      } else {
+        const real txyb = sgn * (g[0] + g[3] * qiu), tyyb = sgn * (g[1] + g[3] * qiv),
+                   tyzb = sgn * (g[2] + g[3] * qiw);
+        const real qiub = sgn * g[3] * txy * h, qivb = sgn * g[3] * tyy * h, qiwb = sgn * g[3] * tyz * h;
+        Sb[1][s][1][0] += qiub; Sb[1][s - 1][1][0] += qiub;
+        Sb[1][s][1][1] += qivb; Sb[1][s - 1][1][1] += qivb;
+        Sb[1][s][1][2] += qiwb; Sb[1][s - 1][1][2] += qiwb;
+        const real dvdyb = f43 * tyyb * oody, dudxb = -f23 * tyyb * q4 * oodx, dwdzb = -f23 * tyyb * q4 * oodz;
+        const real dvdxb = txyb * q4 * oodx, dudyb = txyb * oody;
+        const real dvdzb = tyzb * q4 * oodz, dwdyb = tyzb * oody;
+        Sb[1][s][1][0] += dudyb; Sb[1][s - 1][1][0] -= dudyb;
+        Sb[1][s][1][1] += dvdyb; Sb[1][s - 1][1][1] -= dvdyb;
+        Sb[1][s][1][2] += dwdyb; Sb[1][s - 1][1][2] -= dwdyb;
+        Sb[1][s][2][0] += dudxb; Sb[1][s][0][0] -= dudxb; Sb[1][s - 1][2][0] += dudxb; Sb[1][s - 1][0][0] -= dudxb;
+        Sb[1][s][2][1] += dvdxb; Sb[1][s][0][1] -= dvdxb; Sb[1][s - 1][2][1] += dvdxb; Sb[1][s - 1][0][1] -= dvdxb;
+        Sb[2][s][1][2] += dwdzb; Sb[0][s][1][2] -= dwdzb; Sb[2][s - 1][1][2] += dwdzb; Sb[0][s - 1][1][2] -= dwdzb;
+        Sb[2][s][1][1] += dvdzb; Sb[0][s][1][1] -= dvdzb; Sb[2][s - 1][1][1] += dvdzb; Sb[0][s - 1][1][1] -= dvdzb;
+      }
+    }
+    // ---- Z, with the reference's index and component mix-ups
+    // (Viscosity.h:122-133): S(s,1,1) and S(s-1,1,1)
+    {
+      const real qiu = h * (S[s][1][1][0] + S[s - 1][1][1][0]);
+      const real qiv = h * (S[s][1][1][1] + S[s - 1][1][1][1]);
+      const real qiw = h * (S[s][1][1][2] + S[s - 1][1][1][2]);
+      const real dudz = oodz * (S[s][1][1][0] - S[s - 1][1][1][0]);
+      const real dvdz = oodz * (S[s][1][1][1] - S[s - 1][1][1][1]);
+      const real dwdz = oodz * (S[s][1][1][2] - S[s - 1][1][1][2]);
+      const real dudx = q4 * oodx * (S[s][1][2][0] - S[1][s][0][0] + S[s - 1][1][2][0] - S[1][s - 1][0][0]);
+      const real dwdx = q4 * oodx * (S[s][1][2][1] - S[1][s][0][2] + S[s - 1][1][2][1] - S[1][s - 1][0][2]);
+      const real dvdy = q4 * oodz * (S[s][2][1][2] - S[s][0][1][1] + S[s - 1][2][1][2] - S[s - 1][0][1][1]);
+      const real dwdy = q4 * oodz * (S[s][2][1][1] - S[s][0][1][2] + S[s - 1][2][1][1] - S[s - 1][0][1][2]);
+      const real tzz = f43 * dwdz - f23 * (dudx + dvdy);
+      const real txz = dudz + dwdx;
+      const real tyz = dvdz + dwdy;
+      if (!REV) {
+        out[0] += sgn * txz; out[1] += sgn * tyz; out[2] += sgn * tzz;
+        out[3] += sgn * (txz * qiu + tyz * qiv + tzz * qiw);
+      } else {
+        const real txzb = sgn * (g[0] + g[3] * qiu), tyzb = sgn * (g[1] + g[3] * qiv),
+                   tzzb = sgn * (g[2] + g[3] * qiw);
+        const real qiub = sgn * g[3] * txz * h, qivb = sgn * g[3] * tyz * h, qiwb = sgn * g[3] * tzz * h;
+        Sb[s][1][1][0] += qiub; Sb[s - 1][1][1][0] += qiub;
+        Sb[s][1][1][1] += qivb; Sb[s - 1][1][1][1] += qivb;
+        Sb[s][1][1][2] += qiwb; Sb[s - 1][1][1][2] += qiwb;
+        const real dwdzb = f43 * tzzb * oodz, dudxb = -f23 * tzzb * q4 * oodx, dvdyb = -f23 * tzzb * q4 * oodz;
+        const real dudzb = txzb * oodz, dwdxb = txzb * q4 * oodx;
+        const real dvdzb = tyzb * oodz, dwdyb = tyzb * q4 * oodz;
+        Sb[s][1][1][0] += dudzb; Sb[s - 1][1][1][0] -= dudzb;
+        Sb[s][1][1][1] += dvdzb; Sb[s - 1][1][1][1] -= dvdzb;
+        Sb[s][1][1][2] += dwdzb; Sb[s - 1][1][1][2] -= dwdzb;
+        Sb[s][1][2][0] += dudxb; Sb[1][s][0][0] -= dudxb; Sb[s - 1][1][2][0] += dudxb; Sb[1][s - 1][0][0] -= dudxb;
+        Sb[s][1][2][1] += dwdxb; Sb[1][s][0][2] -= dwdxb; Sb[s - 1][1][2][1] += dwdxb; Sb[1][s - 1][0][2] -= dwdxb;
+        Sb[s][2][1][2] += dvdyb; Sb[s][0][1][1] -= dvdyb; Sb[s - 1][2][1][2] += dvdyb; Sb[s - 1][0][1][1] -= dvdyb;
+        Sb[s][2][1][1] += dwdyb; Sb[s][0][1][2] -= dwdyb; Sb[s - 1][2][1][1] += dwdyb; Sb[s - 1][0][1][2] -= dwdyb;
+      }
+    }
+  }
+}
+
+template <typename real, bool TC, bool VISC>
+__global__ void __launch_bounds__(256)
+k_adj_diffusion(const real* __restrict__ U, const real* __restrict__ v, real sigma, real* Qb,
+                double dt, DevParams P, double* grad) {
+  const long long n = P.sf;
+  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  double gk = 0.0, gm = 0.0;
+  if (t < n) {
+    const int i = (int)(t % P.Nx), j = (int)((t / P.Nx) % P.Ny), k = (int)(t / P.sz);
+    const real gm1 = (real)P.gm1;
+    const real rdt = (real)dt;
+    if (TC) {
+      // E += dt/dx (FR - FL) + dt/dy (FD - FU) + dt/dz (FB - FF), F = kf dT / d
+      // (ThermalConduction.h:47-95); all terms are linear in kf
+      const real Eb = sigma * v[t + 4 * P.sf];
+      real T[7];
+      long long src[7];
+      const int off[7][3] = {{0, 0, 0}, {-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+#pragma unroll
+      for (int o = 0; o < 7; ++o) {
+        bool f;
+        const int si = map_coord(i + off[o][0], P.Nx, P.bcx, f);
+        const int sj = map_coord(j + off[o][1], P.Ny, P.bcy, f);
+        const int sk = map_coord(k + off[o][2], P.Nz, P.bcz, f);
+        src[o] = si + (long long)sj * P.sy + (long long)sk * P.sz;
+        real q[5];
+        load_prim_arr(U, src[o], P.sf, gm1, q);
+        T[o] = q[4] / q[0];
+      }
+      const real ax = rdt / (real)(P.dx * P.dx), ay = rdt / (real)(P.dy * P.dy),
+                 az = rdt / (real)(P.dz * P.dz);
+      // coefficients of dE/dT per stencil entry, per unit kf
+      real c[7] = {-(real)2 * (ax + ay + az), ax, ax, ay, ay, az, az};
+      real fixedE = 0;  // kf-proportional part that does not depend on T
+      if (k == 0 && P.bctc_zmin != FV3D_BCTC_NONE) {
+        // FF is replaced: the -FF term (coefficient -dt/dz) changes
+        c[0] += az;  // undo the default FF = (TC - TF)/dz
+        c[5] -= az;
+        if (P.bctc_zmin == FV3D_BCTC_FIXED_TEMPERATURE) {
+          c[0] -= (real)2 * az;
+          fixedE += (real)2 * az * (real)P.bctc_zmin_value;
+        } else if (P.bctc_zmin == FV3D_BCTC_FIXED_GRADIENT) {
+          fixedE -= rdt / (real)P.dz * (real)P.bctc_zmin_value;
+        }
+      }
+      if (k == P.Nz - 1 && P.bctc_zmax != FV3D_BCTC_NONE) {
+        c[0] += az;
+        c[6] -= az;
+        if (P.bctc_zmax == FV3D_BCTC_FIXED_TEMPERATURE) {
+          c[0] -= (real)2 * az;
+          fixedE += (real)2 * az * (real)P.bctc_zmax_value;
+        } else if (P.bctc_zmax == FV3D_BCTC_FIXED_GRADIENT) {
+          fixedE += rdt / (real)P.dz * (real)P.bctc_zmax_value;
+        }
+      }
+      real dE1 = fixedE;
+#pragma unroll
+      for (int o = 0; o < 7; ++o) dE1 += c[o] * T[o];
+      gk = (double)Eb * (double)dE1;
+      const real kf = (real)P.kface;
+#pragma unroll
+      for (int o = 0; o < 7; ++o) atomic_add(&Qb[src[o] + 5 * P.sf], Eb * kf * c[o]);
+    }
+    if (VISC) {
+      real S[3][3][3][3], Sb[3][3][3][3];
+      long long src[3][3][3];
+      bool fl[3][3][3][3];
+#pragma unroll
+      for (int dk = 0; dk < 3; ++dk)
+#pragma unroll
+        for (int dj = 0; dj < 3; ++dj)
+#pragma unroll
+          for (int di = 0; di < 3; ++di) {
+            bool fx, fy, fz;
+            const int si = map_coord(i + di - 1, P.Nx, P.bcx, fx);
+            const int sj = map_coord(j + dj - 1, P.Ny, P.bcy, fy);
+            const int sk = map_coord(k + dk - 1, P.Nz, P.bcz, fz);
+            const long long s = si + (long long)sj * P.sy + (long long)sk * P.sz;
+            src[dk][dj][di] = s;
+            const real r = U[s];
+            S[dk][dj][di][0] = U[s + P.sf] / r;
+            S[dk][dj][di][1] = U[s + 2 * P.sf] / r;
+            S[dk][dj][di][2] = U[s + 3 * P.sf] / r;
+            fl[dk][dj][di][0] = fx; fl[dk][dj][di][1] = fy; fl[dk][dj][di][2] = fz;
+            if (fx) S[dk][dj][di][0] = -S[dk][dj][di][0];
+            if (fy) S[dk][dj][di][1] = -S[dk][dj][di][1];
+            if (fz) S[dk][dj][di][2] = -S[dk][dj][di][2];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) Sb[dk][dj][di][c] = 0;
+          }
+      const real oodx = (real)1 / (real)P.dx, oody = (real)1 / (real)P.dy, oodz = (real)1 / (real)P.dz;
+      real vb[4], out[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) vb[c] = sigma * v[t + (c + 1) * P.sf];
+      visc_cell<real, false>(S, oodx, oody, oodz, vb, out, Sb);
+      double gmu = 0.0;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) gmu += (double)vb[c] * (double)out[c];
+      gm = gmu * dt;
+      real g[4];
+      const real mdt = (real)P.mu * rdt;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) g[c] = vb[c] * mdt;
+      visc_cell<real, true>(S, oodx, oody, oodz, g, out, Sb);
+#pragma unroll
+      for (int dk = 0; dk < 3; ++dk)
+#pragma unroll
+        for (int dj = 0; dj < 3; ++dj)
+#pragma unroll
+          for (int di = 0; di < 3; ++di)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              const real b = fl[dk][dj][di][c] ? -Sb[dk][dj][di][c] : Sb[dk][dj][di][c];
+              atomic_add(&Qb[src[dk][dj][di] + (c + 1) * P.sf], b);
+            }
+    }
+  }
+  // parameter gradients: wave sums, then one atomic per wave
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    gk += __shfl_xor(gk, off);
+    gm += __shfl_xor(gm, off);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (TC && gk != 0.0) unsafeAtomicAdd(&grad[0], gk);
+    if (VISC && gm != 0.0) unsafeAtomicAdd(&grad[1], gm);
+  }
+}
+
+// out = sigma*v + J_c2p(U)^T Qb + alpha*extra; Qb cleared
+template <typename real>
+__global__ void __launch_bounds__(256)
+k_adj_finalize(const real* __restrict__ U, const real* __restrict__ v, real sigma,
+               const real* __restrict__ extra, real alpha, real* Qb, real* out, DevParams P) {
+  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (t >= P.sf) return;
+  const long long sf = P.sf;
+  const real gm1 = (real)P.gm1;
+  real q[5];
+  load_prim_arr(U, t, sf, gm1, q);
+  real qb[5];
+#pragma unroll
+  for (int n = 0; n < 5; ++n) qb[n] = Qb[t + n * sf];
+  const real Tb = Qb[t + 5 * sf];
+  // T = p / rho
+  qb[4] += Tb / q[0];
+  qb[0] -= Tb * q[4] / (q[0] * q[0]);
+  // consToPrim (States.h:34-45) reversed
+  const real r = q[0], u = q[1], vv = q[2], w = q[3];
+  const real Ub1 = qb[1] / r - qb[4] * gm1 * u;
+  const real Ub2 = qb[2] / r - qb[4] * gm1 * vv;
+  const real Ub3 = qb[3] / r - qb[4] * gm1 * w;
+  const real Ub4 = qb[4] * gm1;
+  const real Ub0 = qb[0] - (qb[1] * u + qb[2] * vv + qb[3] * w) / r +
+                   qb[4] * gm1 * (real)0.5 * (u * u + vv * vv + w * w);
+  const real Ub[5] = {Ub0, Ub1, Ub2, Ub3, Ub4};
+#pragma unroll
+  for (int n = 0; n < 5; ++n) {
+    real o = sigma * v[t + n * sf] + Ub[n];
+    if (extra) o += alpha * extra[t + n * sf];
+    out[t + n * sf] = o;
+  }
+#pragma unroll
+  for (int n = 0; n < 6; ++n) Qb[t + n * sf] = (real)0;
+}
+
+template <typename real, int DIR, int RECON, int RS>
+static hipError_t adj_faces_dir(const AdjLaunch<real>& L, hipStream_t s) {
+  const DevParams& P = L.P;
+  const long long nf = (long long)(P.Nx + (DIR == FV3D_IX)) * (P.Ny + (DIR == FV3D_IY)) *
+                       (P.Nz + (DIR == FV3D_IZ));
+  const int bs = 256;
+  hipLaunchKernelGGL((k_adj_faces<real, DIR, RECON, RS>), dim3((unsigned)((nf + bs - 1) / bs)),
+                     dim3(bs), 0, s, L.U, L.v, L.sigma, L.Qb, L.dt, P);
+  return hipGetLastError();
+}
+
+template <typename real, int RECON, int RS>
+static hipError_t adj_faces(const AdjLaunch<real>& L, hipStream_t s) {
+  hipError_t e = adj_faces_dir<real, FV3D_IX, RECON, RS>(L, s);
+  if (e == hipSuccess) e = adj_faces_dir<real, FV3D_IY, RECON, RS>(L, s);
+  if (e == hipSuccess) e = adj_faces_dir<real, FV3D_IZ, RECON, RS>(L, s);
+  return e;
+}
+
+// one adjoint stage: Qb must be zero on entry and is zero on return
+template <typename real>
+static hipError_t launch_adjoint_stage_t(const AdjLaunch<real>& L, hipStream_t s) {
+  const bool plm = L.recon == FV3D_PLM, hll = L.riemann == FV3D_HLL;
+  hipError_t e;
+  if (plm && hll) e = adj_faces<real, FV3D_PLM, FV3D_HLL>(L, s);
+  else if (plm) e = adj_faces<real, FV3D_PLM, FV3D_HLLC>(L, s);
+  else if (hll) e = adj_faces<real, FV3D_PCM, FV3D_HLL>(L, s);
+  else e = adj_faces<real, FV3D_PCM, FV3D_HLLC>(L, s);
+  if (e != hipSuccess) return e;
+  const int bs = 256;
+  const unsigned nb = (unsigned)((L.P.sf + bs - 1) / bs);
+  if (L.tc && L.visc)
+    hipLaunchKernelGGL((k_adj_diffusion<real, true, true>), dim3(nb), dim3(bs), 0, s, L.U, L.v,
+                       L.sigma, L.Qb, L.dt, L.P, L.grad);
+  else if (L.tc)
+    hipLaunchKernelGGL((k_adj_diffusion<real, true, false>), dim3(nb), dim3(bs), 0, s, L.U, L.v,
+                       L.sigma, L.Qb, L.dt, L.P, L.grad);
+  else if (L.visc)
+    hipLaunchKernelGGL((k_adj_diffusion<real, false, true>), dim3(nb), dim3(bs), 0, s, L.U, L.v,
+                       L.sigma, L.Qb, L.dt, L.P, L.grad);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_adj_finalize<real>), dim3(nb), dim3(bs), 0, s, L.U, L.v, L.sigma,
+                     L.extra, L.alpha, L.Qb, L.out, L.P);
+  return hipGetLastError();
+}
+
+}  // namespace fv3d_hip

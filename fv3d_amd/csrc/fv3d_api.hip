@@ -21,34 +21,9 @@
 #include <vector>
 #include <algorithm>
 
-#include "fv3d_hip.h"
-#include "fv3d_host.h"
-#include "fv3d_kernels.h"
+#include "fv3d_ctx.h"
 
 using namespace fv3d_hip;
-
-struct fv3d_ctx {
-  fv3d_params p;
-  int precision = FV3D_FP64;
-  int device = 0;
-  hipStream_t stream = nullptr;
-  void* A = nullptr;
-  void* B = nullptr;
-  void* C = nullptr;    // FP32 compensated RK2: U^{n+1} target
-  void* Cmp = nullptr;  // FP32 compensated RK2: Kahan compensation
-  size_t elem = sizeof(double);
-  size_t state_bytes = 0;
-  bool stage_is_inc = false;  // B holds the stage-1 increment K (FP32 compensated RK2)
-  void* stage_base = nullptr; // U^n of the last step, when B holds K
-  unsigned long long* ring = nullptr;  // 3 slots
-  int* health = nullptr;
-  double* qbuf = nullptr;  // ghosted SoA primitive staging, allocated on first use
-  long step = 0;
-  bool have_state = false;
-  bool have_stage = false;  // B holds the reference's post-update Q source
-  DevParams dp;
-  std::string err;
-};
 
 static thread_local std::string g_err;
 
@@ -59,22 +34,15 @@ int record_error(int code, const std::string& msg) {
 }
 }  // namespace fv3d_hip
 
-static int fail(fv3d_ctx* c, int code, const std::string& msg) {
+int ctx_fail(fv3d_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
   g_err = msg;
   return code;
 }
 
-#define HIPCHK(ctx, expr)                                                              \
-  do {                                                                                 \
-    hipError_t e_ = (expr);                                                            \
-    if (e_ != hipSuccess)                                                              \
-      return fail((ctx), FV3D_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-  } while (0)
+static int fail(fv3d_ctx* c, int code, const std::string& msg) { return ctx_fail(c, code, msg); }
 
-static bool kahan(const fv3d_ctx* c) {
-  return c->precision == FV3D_FP32 && c->p.time_stepping == FV3D_TS_RK2;
-}
+static bool kahan(const fv3d_ctx* c) { return ctx_kahan(c); }
 
 extern "C" {
 
@@ -185,7 +153,7 @@ static int state_prim_to_host(fv3d_ctx* c, const void* U, const void* K, double*
   return FV3D_OK;
 }
 
-static int reset_dt(fv3d_ctx* c) {
+int ctx_reset_dt(fv3d_ctx* c) {
   HIPCHK(c, launch_fill_zero_u64(c->ring, 4, c->stream));
   HIPCHK(c, hipMemsetAsync(c->health, 0, sizeof(int), c->stream));
   if (c->precision == FV3D_FP64)
@@ -216,7 +184,7 @@ int fv3d_set_prim(fv3d_ctx* c, const double* Q, int with_ghosts) {
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   hipFree(d);
   if (e != hipSuccess) return fail(c, FV3D_EHIP, std::string("set_prim: ") + hipGetErrorString(e));
-  return reset_dt(c);
+  return ctx_reset_dt(c);
 }
 
 int fv3d_set_cons(fv3d_ctx* c, const double* U, int with_ghosts) {
@@ -232,7 +200,7 @@ int fv3d_set_cons(fv3d_ctx* c, const double* U, int with_ghosts) {
   }
   HIPCHK(c, hipMemcpyAsync(c->A, src, c->state_bytes, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  return reset_dt(c);
+  return ctx_reset_dt(c);
 }
 
 int fv3d_get_cons(fv3d_ctx* c, double* U, int with_ghosts) {

@@ -32,6 +32,7 @@ struct DevParams {
   double mu;
   double CFL, inv_tc, inv_visc; // dt inputs; inv_tc/inv_visc are constants of the run
   int tiles_x, tiles_y, tiles_z, nzc;
+  int dt_direct;                // the dt slot holds dt itself (a frozen dt sequence)
 };
 
 template <typename real>
@@ -230,6 +231,7 @@ __device__ __forceinline__ unsigned long long double_as_u64(double d) {
 
 // dt = CFL / max(inv_dt_hyp, inv_dt_par_tc, inv_dt_par_visc)  (ComputeDt.h:62)
 __device__ __forceinline__ double dt_from_inv(double ih, const DevParams& P) {
+  if (P.dt_direct) return ih;
   double m = ih;
   if (m < P.inv_tc) m = P.inv_tc;
   if (m < P.inv_visc) m = P.inv_visc;

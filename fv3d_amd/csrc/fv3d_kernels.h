@@ -62,6 +62,28 @@ template <typename real>
 hipError_t launch_batch_stage(const BatchLaunch& L, hipStream_t s);
 hipError_t launch_batch_advance(long long* step, unsigned* counters, int n, hipStream_t s);
 
+// ---- discrete adjoint of one stage (fv3d_adjoint_impl.h) ----
+// out = sigma*v + dt J_R(U)^T (sigma*v) + alpha*extra, where the stage is
+// S(U) = U + dt R(U) with dt frozen; grad[0..1] += (sigma*v) . dS/dkappa,
+// (sigma*v) . dS/dmu.  Qb: [6][N] scratch, zero on entry and on return.
+template <typename real>
+struct AdjLaunch {
+  const real* U;
+  const real* v;
+  real sigma;
+  const real* extra;  // may be nullptr
+  real alpha;
+  real* Qb;
+  real* out;
+  double dt;
+  double* grad;
+  DevParams P;
+  int recon, riemann;
+  bool tc, visc;
+};
+template <typename real>
+hipError_t launch_adjoint_stage(const AdjLaunch<real>& L, hipStream_t s);
+
 // cons -> prim over the domain of U (or of U + K when K is given), written
 // into a ghosted SoA array [5][Ntz][Nty][Ntx] (ghosts untouched)
 template <typename real>

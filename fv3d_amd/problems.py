@@ -214,6 +214,63 @@ def rt_ensemble(base, nbox: int, seed: int = 12345, sides=(32, 48, 64, 96)):
     return out
 
 
+def _spectral_field(rng, shape, slope):
+    """Real Gaussian random field with power spectrum P(k) ~ k^slope on a
+    periodic grid (k in units of the box), zero mean, unit rms."""
+    white = rng.standard_normal(shape)
+    f = np.fft.rfftn(white)
+    kz = np.fft.fftfreq(shape[0]) * shape[0]
+    ky = np.fft.fftfreq(shape[1]) * shape[1]
+    kx = np.fft.rfftfreq(shape[2]) * shape[2]
+    K = np.sqrt(kz[:, None, None] ** 2 + ky[None, :, None] ** 2 + kx[None, None, :] ** 2)
+    amp = np.zeros_like(K)
+    amp[K > 0] = K[K > 0] ** (slope / 2.0)
+    g = np.fft.irfftn(f * amp, s=shape, axes=(0, 1, 2))
+    g -= g.mean()
+    return g / np.sqrt(np.mean(g * g))
+
+
+def _solenoidal_velocity(rng, shape, slope):
+    """Divergence-free random velocity (u, v, w) on a periodic grid with
+    spectrum ~ k^slope, projected in Fourier space, unit total rms."""
+    kz = np.fft.fftfreq(shape[0]) * shape[0]
+    ky = np.fft.fftfreq(shape[1]) * shape[1]
+    kx = np.fft.rfftfreq(shape[2]) * shape[2]
+    KZ, KY, KX = np.meshgrid(kz, ky, kx, indexing="ij")
+    K2 = KX ** 2 + KY ** 2 + KZ ** 2
+    amp = np.zeros_like(K2)
+    amp[K2 > 0] = K2[K2 > 0] ** (slope / 4.0)
+    comps = [np.fft.rfftn(rng.standard_normal(shape)) * amp for _ in range(3)]  # x, y, z
+    div = (KX * comps[0] + KY * comps[1] + KZ * comps[2])
+    with np.errstate(invalid="ignore", divide="ignore"):
+        scale = np.where(K2 > 0, div / K2, 0.0)
+    comps = [comps[0] - KX * scale, comps[1] - KY * scale, comps[2] - KZ * scale]
+    vel = [np.fft.irfftn(c, s=shape, axes=(0, 1, 2)) for c in comps]
+    rms = np.sqrt(np.mean(vel[0] ** 2 + vel[1] ** 2 + vel[2] ** 2))
+    return [v / rms for v in vel]
+
+
+def turbulent_box(p, seed: int = 12345, rho_amp: float = 0.05, v_rms: float = 0.1,
+                  slope: float = -2.0, p_amp: float = 0.0):
+    """Periodic box for the adjoint workload: rho = 1 + rho_amp * GRF
+    (P(k) ~ k^slope, unit rms), p = 1 (+ p_amp * an independent GRF),
+    solenoidal random velocity of rms v_rms, all drawn from numpy's PCG64
+    seeded with `seed`."""
+    rng = np.random.default_rng(seed)
+    shape = (p.Nz, p.Ny, p.Nx)
+    g = _spectral_field(rng, shape, slope)
+    u, v, w = _solenoidal_velocity(rng, shape, slope)
+    gp = _spectral_field(rng, shape, slope) if p_amp else 0.0
+    Q = _empty(p)
+    d = _dom(Q, p)
+    d[..., 0] = 1.0 + rho_amp * g
+    d[..., 1] = v_rms * u
+    d[..., 2] = v_rms * v
+    d[..., 3] = v_rms * w
+    d[..., 4] = 1.0 + p_amp * gp
+    return Q
+
+
 def initial_condition(cfg) -> np.ndarray:
     """Q for a RunConfig (read_inifile) -- InitFunctor::init (Init.h:253-278)."""
     p = cfg.params

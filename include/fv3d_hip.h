@@ -104,6 +104,34 @@ int fv3d_riemann(int solver, double gamma0, long n, const double* qL, const doub
  * calibrates the profiler's byte counters (nbytes read + nbytes written). */
 int fv3d_stream_copy(size_t nbytes, int reps, float* ms_per_copy);
 
+/* ---- frozen-dt runs and the discrete adjoint (no counterpart in the
+ * reference) ----
+ *
+ * fv3d_forward_frozen advances the context's state by nsteps steps of
+ * UpdateFunctor::update (Update.h:180-206) with the given dt sequence
+ * instead of computeDt's (the adjoint's forward map); *ms (may be NULL)
+ * gets the device time.  Afterwards the context restarts its step count and
+ * dt ring from the new state, as after fv3d_set_cons.  FP32 contexts run
+ * the uncompensated FP32 RK2.
+ *
+ * fv3d_adjoint_run differentiates J(U^n) for the run of nsteps frozen-dt
+ * steps that starts from the context's current state: given dJ_dUT =
+ * dJ/dU^n (conserved, host layout without ghosts (Nz, Ny, Nx, 5)), it
+ * returns dJ_dU0 = dJ/dU^0 (same layout) and, if not NULL, dJ/dkappa and
+ * dJ/dmu (constant conductivity and viscosity).  The forward pass keeps a
+ * checkpoint every K steps; each K-step segment is recomputed in the
+ * backward pass.  Branches (minmod selection, Riemann wave region) are
+ * those of the forward evaluation; the density floor is taken as inactive.
+ * Gravity and the well-balanced variants are rejected (FV3D_EINVAL).  The
+ * context's state is unchanged.
+ * fv3d_adjoint_times gives the device times of the last fv3d_adjoint_run:
+ * ms[0] forward pass, ms[1] backward pass (recomputation included),
+ * ms[2] total. */
+int fv3d_forward_frozen(fv3d_ctx* ctx, int nsteps, const double* dts, float* ms);
+int fv3d_adjoint_run(fv3d_ctx* ctx, int nsteps, int K, const double* dts, const double* dJ_dUT,
+                     double* dJ_dU0, double* dJ_dkappa, double* dJ_dmu);
+int fv3d_adjoint_times(const fv3d_ctx* ctx, float ms[3]);
+
 /* ---- ensembles of independent boxes (no counterpart in the reference) ----
  *
  * nbox boxes, each a complete fv3d run with its own fv3d_params (sizes,
