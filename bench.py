@@ -198,6 +198,91 @@ def ensemble_form(F, nbox, steps, tune_cache=None):
             "l1_rho_fp32_vs_fp64_max": max(l1), "l1_rho_fp32_vs_fp64_mean": float(np.mean(l1))}
 
 
+def adjoint_form(F, n=128, steps=200, K=20, ndirs=16, precision=None, fd=True):
+    """The discrete adjoint workload: n^3 periodic box, gamma 5/3, PLM +
+    HLLC, SSP-RK2, CFL 0.4, conduction kappa=0.02 and viscosity mu=0.01;
+    rho = 1 + 0.05 GRF (P(k) ~ k^-2), p = 1, solenoidal velocity of rms 0.1.
+    J = sum((rho_T - rho_target)^2) after `steps` steps with the dt sequence
+    of the ordinary run frozen; the target is the run from the next seed.
+    Reports the adjoint's device time (checkpointed forward pass + backward
+    pass) over the frozen forward run's, and central differences
+    (eps = 1e-6) of J along `ndirs` unit-norm random directions of the
+    initial density field against the adjoint's dJ/drho0, plus dJ/dkappa and
+    dJ/dmu against their own central differences."""
+    from fv3d_amd import problems
+    precision = F.FP64 if precision is None else precision
+    p = F.Params.defaults().replace(
+        Nx=n, Ny=n, Nz=n, gamma0=5.0 / 3.0, reconstruction=F.PLM, riemann_solver=F.HLLC,
+        time_stepping=F.TS_RK2, CFL=0.4, boundary_x=F.BC_PERIODIC, boundary_y=F.BC_PERIODIC,
+        boundary_z=F.BC_PERIODIC, thermal_conductivity_active=1, kappa=0.02, viscosity_active=1,
+        mu=0.01)
+    Q0 = problems.turbulent_box(p, seed=12345)
+    sim = F.Simulation(p, precision)
+    sim.set_prim(Q0)
+    dts = []
+    for _ in range(steps):
+        dts.append(sim.compute_dt()[0])
+        sim.step()
+    dts = np.array(dts)
+    sim.set_prim(problems.turbulent_box(p, seed=12346))
+    sim.forward_frozen(dts)
+    target = sim.get_prim(with_ghosts=False)[..., 0].copy()
+    d0 = Q0[p.kbeg:p.kend, p.jbeg:p.jend, p.ibeg:p.iend]
+    vel = d0[..., 1:4]
+
+    def J_of(rho0, q=p):
+        prim = d0.copy()
+        prim[..., 0] = rho0
+        s2 = sim if q is p else F.Simulation(q, precision)
+        s2.set_prim(np.ascontiguousarray(prim), with_ghosts=False)
+        ms = s2.forward_frozen(dts)
+        r = s2.get_prim(with_ghosts=False)[..., 0]
+        if s2 is not sim:
+            s2.close()
+        return float(np.sum((r - target) ** 2)), r, ms
+
+    J0, rT, _ = J_of(d0[..., 0])
+    fwd_ms = []
+    for _ in range(2):
+        sim.set_prim(Q0)
+        fwd_ms.append(sim.forward_frozen(dts))
+    sim.set_prim(Q0)
+    lamT = np.zeros(p.shape_domain)
+    lamT[..., 0] = 2.0 * (rT - target)
+    lam0, dk, dm = sim.adjoint(dts, lamT, K)
+    t_fwd_pass, t_bwd, t_adj = sim.adjoint_times()
+    g = lam0[..., 0] + (lam0[..., 1:4] * vel).sum(-1) + lam0[..., 4] * 0.5 * (vel * vel).sum(-1)
+    print(f"adjoint: forward {min(fwd_ms):.1f} ms, adjoint {t_adj:.1f} ms", file=sys.stderr, flush=True)
+    out = {"grid": [n, n, n], "steps": steps, "checkpoint_every": K,
+           "dtype": "f64" if precision == F.FP64 else "f32",
+           "forward_ms": round(min(fwd_ms), 3), "adjoint_ms": round(t_adj, 3),
+           "adjoint_forward_pass_ms": round(t_fwd_pass, 3), "adjoint_backward_ms": round(t_bwd, 3),
+           "adjoint_over_forward": round(t_adj / min(fwd_ms), 3), "J": J0,
+           "dJ_dkappa": dk, "dJ_dmu": dm}
+    if fd and ndirs > 0:
+        rng = np.random.default_rng(7)
+        eps = 1e-6
+        rel = []
+        for i in range(ndirs):
+            d = rng.standard_normal(g.shape)
+            d /= np.linalg.norm(d)
+            Jp = J_of(d0[..., 0] + eps * d)[0]
+            Jm = J_of(d0[..., 0] - eps * d)[0]
+            fdv = (Jp - Jm) / (2 * eps)
+            ad = float(np.sum(g * d))
+            rel.append(abs(fdv - ad) / abs(ad))
+        print(f"adjoint: FD rho0 rel errors max {max(rel):.2e}", file=sys.stderr, flush=True)
+        out["fd_rho0_rel_err"] = {"eps": eps, "directions": ndirs, "max": max(rel),
+                                  "median": float(np.median(rel))}
+        for key, val, ad in [("kappa", p.kappa, dk), ("mu", p.mu, dm)]:
+            h = eps * val
+            Jp = J_of(d0[..., 0], p.replace(**{key: val + h}))[0]
+            Jm = J_of(d0[..., 0], p.replace(**{key: val - h}))[0]
+            out[f"fd_{key}_rel_err"] = abs((Jp - Jm) / (2 * h) - ad) / abs(ad)
+    sim.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -210,6 +295,9 @@ def main():
     ap.add_argument("--forms", type=int, default=1, help="also time the FP32 forms")
     ap.add_argument("--ensemble", type=int, default=512, help="boxes of the batched ensemble (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=500)
+    ap.add_argument("--adjoint", type=int, default=128, help="grid of the adjoint workload (0: skip)")
+    ap.add_argument("--adjoint-steps", type=int, default=200)
+    ap.add_argument("--adjoint-dirs", type=int, default=16)
     ap.add_argument("--copy-only", action="store_true",
                     help="run only the 1 GiB copy (calibration run for the rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -277,6 +365,9 @@ def main():
         }
     if args.ensemble > 0:
         out.setdefault("forms", {})["fp32_ensemble_rt"] = ensemble_form(F, args.ensemble, args.ensemble_steps)
+    if args.adjoint > 0:
+        out.setdefault("forms", {})["adjoint_fp64"] = adjoint_form(F, args.adjoint, args.adjoint_steps,
+                                                                   ndirs=args.adjoint_dirs)
     if args.cpu_budget > 0:
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_budget)
     else:

@@ -14,6 +14,7 @@
 // the sums over all stages of (adjoint of the stage output) . dS/dtheta.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -156,6 +157,7 @@ int adjoint_run_t(fv3d_ctx* c, int nsteps, int K, const double* dts_dev,
   A.P = c->dp;
   A.Qb = Qb;
   A.grad = grad;
+  A.simple = std::getenv("FV3D_ADJ_SIMPLE") != nullptr;
   for (int cp = ncp - 1; cp >= 0; --cp) {
     const int s0 = cp * K, s1 = std::min(nsteps, s0 + K);
     // recompute the segment: seg0[i] = U^{s0+i}, seg1[i] = U1 of step s0+i

@@ -355,13 +355,13 @@ k_adj_faces(const real* __restrict__ U, const real* __restrict__ v, real sigma, 
 // ---------------------------------------------------------------------------
 
 // The viscous update of one cell, Viscosity.h:28-153 with mu = 1, as a
-// function of its 3x3x3 velocity stencil S[dk][dj][di][c] (c = u, v, w):
+// function of its 3x3x3 velocity stencil S(dk, dj, di, c) (c = u, v, w):
 // out[0..3] = increments of rho u, rho v, rho w, E (before * mu * dt).
-// In reverse mode (REV) it accumulates d(out . g)/dS into Sb instead.
-template <typename real, bool REV>
-__device__ __forceinline__ void visc_cell(const real (&S)[3][3][3][3], real oodx, real oody,
-                                          real oodz, const real g[4], real out[4],
-                                          real (&Sb)[3][3][3][3]) {
+// In reverse mode (REV) it hands d(out . g)/dS to SB(dk, dj, di, c, value)
+// instead, one term at a time.
+template <typename real, bool REV, class SF, class SBF>
+__device__ __forceinline__ void visc_cell(SF&& S, real oodx, real oody, real oodz, const real g[4],
+                                          real out[4], SBF&& SB) {
   const real f43 = (real)(4.0 / 3.0), f23 = (real)(2.0 / 3.0), h = (real)0.5, q4 = (real)0.25;
 #pragma unroll
   for (int n = 0; n < 4; ++n) out[n] = 0;
@@ -370,16 +370,16 @@ __device__ __forceinline__ void visc_cell(const real (&S)[3][3][3][3], real oodx
     const real sgn = s == 1 ? (real)-1 : (real)1;
     // ---- X: S(1,1,s) and S(1,1,s-1), y and z differences around them
     {
-      const real qiu = h * (S[1][1][s][0] + S[1][1][s - 1][0]);
-      const real qiv = h * (S[1][1][s][1] + S[1][1][s - 1][1]);
-      const real qiw = h * (S[1][1][s][2] + S[1][1][s - 1][2]);
-      const real dudx = oodx * (S[1][1][s][0] - S[1][1][s - 1][0]);
-      const real dvdx = oodx * (S[1][1][s][1] - S[1][1][s - 1][1]);
-      const real dwdx = oodx * (S[1][1][s][2] - S[1][1][s - 1][2]);
-      const real dudy = q4 * oody * (S[1][2][s][0] - S[1][0][s][0] + S[1][2][s - 1][0] - S[1][0][s - 1][0]);
-      const real dvdy = q4 * oody * (S[1][2][s][1] - S[1][0][s][1] + S[1][2][s - 1][1] - S[1][0][s - 1][1]);
-      const real dudz = q4 * oodz * (S[2][1][s][0] - S[0][1][s][0] + S[2][1][s - 1][0] - S[0][1][s - 1][0]);
-      const real dwdz = q4 * oodz * (S[2][1][s][2] - S[0][1][s][2] + S[2][1][s - 1][2] - S[0][1][s - 1][2]);
+      const real qiu = h * (S(1, 1, s, 0) + S(1, 1, s - 1, 0));
+      const real qiv = h * (S(1, 1, s, 1) + S(1, 1, s - 1, 1));
+      const real qiw = h * (S(1, 1, s, 2) + S(1, 1, s - 1, 2));
+      const real dudx = oodx * (S(1, 1, s, 0) - S(1, 1, s - 1, 0));
+      const real dvdx = oodx * (S(1, 1, s, 1) - S(1, 1, s - 1, 1));
+      const real dwdx = oodx * (S(1, 1, s, 2) - S(1, 1, s - 1, 2));
+      const real dudy = q4 * oody * (S(1, 2, s, 0) - S(1, 0, s, 0) + S(1, 2, s - 1, 0) - S(1, 0, s - 1, 0));
+      const real dvdy = q4 * oody * (S(1, 2, s, 1) - S(1, 0, s, 1) + S(1, 2, s - 1, 1) - S(1, 0, s - 1, 1));
+      const real dudz = q4 * oodz * (S(2, 1, s, 0) - S(0, 1, s, 0) + S(2, 1, s - 1, 0) - S(0, 1, s - 1, 0));
+      const real dwdz = q4 * oodz * (S(2, 1, s, 2) - S(0, 1, s, 2) + S(2, 1, s - 1, 2) - S(0, 1, s - 1, 2));
       const real txx = f43 * dudx - f23 * (dvdy + dwdz);
       const real txy = dvdx + dudy;
       const real txz = dwdx + dudz;
@@ -390,33 +390,33 @@ __device__ __forceinline__ void visc_cell(const real (&S)[3][3][3][3], real oodx
         const real txxb = sgn * (g[0] + g[3] * qiu), txyb = sgn * (g[1] + g[3] * qiv),
                    txzb = sgn * (g[2] + g[3] * qiw);
         const real qiub = sgn * g[3] * txx * h, qivb = sgn * g[3] * txy * h, qiwb = sgn * g[3] * txz * h;
-        Sb[1][1][s][0] += qiub; Sb[1][1][s - 1][0] += qiub;
-        Sb[1][1][s][1] += qivb; Sb[1][1][s - 1][1] += qivb;
-        Sb[1][1][s][2] += qiwb; Sb[1][1][s - 1][2] += qiwb;
+        SB(1, 1, s, 0, qiub); SB(1, 1, s - 1, 0, qiub);
+        SB(1, 1, s, 1, qivb); SB(1, 1, s - 1, 1, qivb);
+        SB(1, 1, s, 2, qiwb); SB(1, 1, s - 1, 2, qiwb);
         const real dudxb = f43 * txxb * oodx, dvdyb = -f23 * txxb * q4 * oody, dwdzb = -f23 * txxb * q4 * oodz;
         const real dvdxb = txyb * oodx, dudyb = txyb * q4 * oody;
         const real dwdxb = txzb * oodx, dudzb = txzb * q4 * oodz;
-        Sb[1][1][s][0] += dudxb; Sb[1][1][s - 1][0] -= dudxb;
-        Sb[1][1][s][1] += dvdxb; Sb[1][1][s - 1][1] -= dvdxb;
-        Sb[1][1][s][2] += dwdxb; Sb[1][1][s - 1][2] -= dwdxb;
-        Sb[1][2][s][0] += dudyb; Sb[1][0][s][0] -= dudyb; Sb[1][2][s - 1][0] += dudyb; Sb[1][0][s - 1][0] -= dudyb;
-        Sb[1][2][s][1] += dvdyb; Sb[1][0][s][1] -= dvdyb; Sb[1][2][s - 1][1] += dvdyb; Sb[1][0][s - 1][1] -= dvdyb;
-        Sb[2][1][s][0] += dudzb; Sb[0][1][s][0] -= dudzb; Sb[2][1][s - 1][0] += dudzb; Sb[0][1][s - 1][0] -= dudzb;
-        Sb[2][1][s][2] += dwdzb; Sb[0][1][s][2] -= dwdzb; Sb[2][1][s - 1][2] += dwdzb; Sb[0][1][s - 1][2] -= dwdzb;
+        SB(1, 1, s, 0, dudxb); SB(1, 1, s - 1, 0, -(dudxb));
+        SB(1, 1, s, 1, dvdxb); SB(1, 1, s - 1, 1, -(dvdxb));
+        SB(1, 1, s, 2, dwdxb); SB(1, 1, s - 1, 2, -(dwdxb));
+        SB(1, 2, s, 0, dudyb); SB(1, 0, s, 0, -(dudyb)); SB(1, 2, s - 1, 0, dudyb); SB(1, 0, s - 1, 0, -(dudyb));
+        SB(1, 2, s, 1, dvdyb); SB(1, 0, s, 1, -(dvdyb)); SB(1, 2, s - 1, 1, dvdyb); SB(1, 0, s - 1, 1, -(dvdyb));
+        SB(2, 1, s, 0, dudzb); SB(0, 1, s, 0, -(dudzb)); SB(2, 1, s - 1, 0, dudzb); SB(0, 1, s - 1, 0, -(dudzb));
+        SB(2, 1, s, 2, dwdzb); SB(0, 1, s, 2, -(dwdzb)); SB(2, 1, s - 1, 2, dwdzb); SB(0, 1, s - 1, 2, -(dwdzb));
       }
     }
     // ---- Y: S(1,s,1) and S(1,s-1,1)
     {
-      const real qiu = h * (S[1][s][1][0] + S[1][s - 1][1][0]);
-      const real qiv = h * (S[1][s][1][1] + S[1][s - 1][1][1]);
-      const real qiw = h * (S[1][s][1][2] + S[1][s - 1][1][2]);
-      const real dudy = oody * (S[1][s][1][0] - S[1][s - 1][1][0]);
-      const real dvdy = oody * (S[1][s][1][1] - S[1][s - 1][1][1]);
-      const real dwdy = oody * (S[1][s][1][2] - S[1][s - 1][1][2]);
-      const real dudx = q4 * oodx * (S[1][s][2][0] - S[1][s][0][0] + S[1][s - 1][2][0] - S[1][s - 1][0][0]);
-      const real dvdx = q4 * oodx * (S[1][s][2][1] - S[1][s][0][1] + S[1][s - 1][2][1] - S[1][s - 1][0][1]);
-      const real dwdz = q4 * oodz * (S[2][s][1][2] - S[0][s][1][2] + S[2][s - 1][1][2] - S[0][s - 1][1][2]);
-      const real dvdz = q4 * oodz * (S[2][s][1][1] - S[0][s][1][1] + S[2][s - 1][1][1] - S[0][s - 1][1][1]);
+      const real qiu = h * (S(1, s, 1, 0) + S(1, s - 1, 1, 0));
+      const real qiv = h * (S(1, s, 1, 1) + S(1, s - 1, 1, 1));
+      const real qiw = h * (S(1, s, 1, 2) + S(1, s - 1, 1, 2));
+      const real dudy = oody * (S(1, s, 1, 0) - S(1, s - 1, 1, 0));
+      const real dvdy = oody * (S(1, s, 1, 1) - S(1, s - 1, 1, 1));
+      const real dwdy = oody * (S(1, s, 1, 2) - S(1, s - 1, 1, 2));
+      const real dudx = q4 * oodx * (S(1, s, 2, 0) - S(1, s, 0, 0) + S(1, s - 1, 2, 0) - S(1, s - 1, 0, 0));
+      const real dvdx = q4 * oodx * (S(1, s, 2, 1) - S(1, s, 0, 1) + S(1, s - 1, 2, 1) - S(1, s - 1, 0, 1));
+      const real dwdz = q4 * oodz * (S(2, s, 1, 2) - S(0, s, 1, 2) + S(2, s - 1, 1, 2) - S(0, s - 1, 1, 2));
+      const real dvdz = q4 * oodz * (S(2, s, 1, 1) - S(0, s, 1, 1) + S(2, s - 1, 1, 1) - S(0, s - 1, 1, 1));
       const real tyy = f43 * dvdy - f23 * (dudx + dwdz);
       const real txy = dvdx + dudy;
       const real tyz = dvdz + dwdy;
@@ -427,34 +427,34 @@ __device__ __forceinline__ void visc_cell(const real (&S)[3][3][3][3], real oodx
         const real txyb = sgn * (g[0] + g[3] * qiu), tyyb = sgn * (g[1] + g[3] * qiv),
                    tyzb = sgn * (g[2] + g[3] * qiw);
         const real qiub = sgn * g[3] * txy * h, qivb = sgn * g[3] * tyy * h, qiwb = sgn * g[3] * tyz * h;
-        Sb[1][s][1][0] += qiub; Sb[1][s - 1][1][0] += qiub;
-        Sb[1][s][1][1] += qivb; Sb[1][s - 1][1][1] += qivb;
-        Sb[1][s][1][2] += qiwb; Sb[1][s - 1][1][2] += qiwb;
+        SB(1, s, 1, 0, qiub); SB(1, s - 1, 1, 0, qiub);
+        SB(1, s, 1, 1, qivb); SB(1, s - 1, 1, 1, qivb);
+        SB(1, s, 1, 2, qiwb); SB(1, s - 1, 1, 2, qiwb);
         const real dvdyb = f43 * tyyb * oody, dudxb = -f23 * tyyb * q4 * oodx, dwdzb = -f23 * tyyb * q4 * oodz;
         const real dvdxb = txyb * q4 * oodx, dudyb = txyb * oody;
         const real dvdzb = tyzb * q4 * oodz, dwdyb = tyzb * oody;
-        Sb[1][s][1][0] += dudyb; Sb[1][s - 1][1][0] -= dudyb;
-        Sb[1][s][1][1] += dvdyb; Sb[1][s - 1][1][1] -= dvdyb;
-        Sb[1][s][1][2] += dwdyb; Sb[1][s - 1][1][2] -= dwdyb;
-        Sb[1][s][2][0] += dudxb; Sb[1][s][0][0] -= dudxb; Sb[1][s - 1][2][0] += dudxb; Sb[1][s - 1][0][0] -= dudxb;
-        Sb[1][s][2][1] += dvdxb; Sb[1][s][0][1] -= dvdxb; Sb[1][s - 1][2][1] += dvdxb; Sb[1][s - 1][0][1] -= dvdxb;
-        Sb[2][s][1][2] += dwdzb; Sb[0][s][1][2] -= dwdzb; Sb[2][s - 1][1][2] += dwdzb; Sb[0][s - 1][1][2] -= dwdzb;
-        Sb[2][s][1][1] += dvdzb; Sb[0][s][1][1] -= dvdzb; Sb[2][s - 1][1][1] += dvdzb; Sb[0][s - 1][1][1] -= dvdzb;
+        SB(1, s, 1, 0, dudyb); SB(1, s - 1, 1, 0, -(dudyb));
+        SB(1, s, 1, 1, dvdyb); SB(1, s - 1, 1, 1, -(dvdyb));
+        SB(1, s, 1, 2, dwdyb); SB(1, s - 1, 1, 2, -(dwdyb));
+        SB(1, s, 2, 0, dudxb); SB(1, s, 0, 0, -(dudxb)); SB(1, s - 1, 2, 0, dudxb); SB(1, s - 1, 0, 0, -(dudxb));
+        SB(1, s, 2, 1, dvdxb); SB(1, s, 0, 1, -(dvdxb)); SB(1, s - 1, 2, 1, dvdxb); SB(1, s - 1, 0, 1, -(dvdxb));
+        SB(2, s, 1, 2, dwdzb); SB(0, s, 1, 2, -(dwdzb)); SB(2, s - 1, 1, 2, dwdzb); SB(0, s - 1, 1, 2, -(dwdzb));
+        SB(2, s, 1, 1, dvdzb); SB(0, s, 1, 1, -(dvdzb)); SB(2, s - 1, 1, 1, dvdzb); SB(0, s - 1, 1, 1, -(dvdzb));
       }
     }
     // ---- Z, with the reference's index and component mix-ups
     // (Viscosity.h:122-133): S(s,1,1) and S(s-1,1,1)
     {
-      const real qiu = h * (S[s][1][1][0] + S[s - 1][1][1][0]);
-      const real qiv = h * (S[s][1][1][1] + S[s - 1][1][1][1]);
-      const real qiw = h * (S[s][1][1][2] + S[s - 1][1][1][2]);
-      const real dudz = oodz * (S[s][1][1][0] - S[s - 1][1][1][0]);
-      const real dvdz = oodz * (S[s][1][1][1] - S[s - 1][1][1][1]);
-      const real dwdz = oodz * (S[s][1][1][2] - S[s - 1][1][1][2]);
-      const real dudx = q4 * oodx * (S[s][1][2][0] - S[1][s][0][0] + S[s - 1][1][2][0] - S[1][s - 1][0][0]);
-      const real dwdx = q4 * oodx * (S[s][1][2][1] - S[1][s][0][2] + S[s - 1][1][2][1] - S[1][s - 1][0][2]);
-      const real dvdy = q4 * oodz * (S[s][2][1][2] - S[s][0][1][1] + S[s - 1][2][1][2] - S[s - 1][0][1][1]);
-      const real dwdy = q4 * oodz * (S[s][2][1][1] - S[s][0][1][2] + S[s - 1][2][1][1] - S[s - 1][0][1][2]);
+      const real qiu = h * (S(s, 1, 1, 0) + S(s - 1, 1, 1, 0));
+      const real qiv = h * (S(s, 1, 1, 1) + S(s - 1, 1, 1, 1));
+      const real qiw = h * (S(s, 1, 1, 2) + S(s - 1, 1, 1, 2));
+      const real dudz = oodz * (S(s, 1, 1, 0) - S(s - 1, 1, 1, 0));
+      const real dvdz = oodz * (S(s, 1, 1, 1) - S(s - 1, 1, 1, 1));
+      const real dwdz = oodz * (S(s, 1, 1, 2) - S(s - 1, 1, 1, 2));
+      const real dudx = q4 * oodx * (S(s, 1, 2, 0) - S(1, s, 0, 0) + S(s - 1, 1, 2, 0) - S(1, s - 1, 0, 0));
+      const real dwdx = q4 * oodx * (S(s, 1, 2, 1) - S(1, s, 0, 2) + S(s - 1, 1, 2, 1) - S(1, s - 1, 0, 2));
+      const real dvdy = q4 * oodz * (S(s, 2, 1, 2) - S(s, 0, 1, 1) + S(s - 1, 2, 1, 2) - S(s - 1, 0, 1, 1));
+      const real dwdy = q4 * oodz * (S(s, 2, 1, 1) - S(s, 0, 1, 2) + S(s - 1, 2, 1, 1) - S(s - 1, 0, 1, 2));
       const real tzz = f43 * dwdz - f23 * (dudx + dvdy);
       const real txz = dudz + dwdx;
       const real tyz = dvdz + dwdy;
@@ -465,19 +465,19 @@ __device__ __forceinline__ void visc_cell(const real (&S)[3][3][3][3], real oodx
         const real txzb = sgn * (g[0] + g[3] * qiu), tyzb = sgn * (g[1] + g[3] * qiv),
                    tzzb = sgn * (g[2] + g[3] * qiw);
         const real qiub = sgn * g[3] * txz * h, qivb = sgn * g[3] * tyz * h, qiwb = sgn * g[3] * tzz * h;
-        Sb[s][1][1][0] += qiub; Sb[s - 1][1][1][0] += qiub;
-        Sb[s][1][1][1] += qivb; Sb[s - 1][1][1][1] += qivb;
-        Sb[s][1][1][2] += qiwb; Sb[s - 1][1][1][2] += qiwb;
+        SB(s, 1, 1, 0, qiub); SB(s - 1, 1, 1, 0, qiub);
+        SB(s, 1, 1, 1, qivb); SB(s - 1, 1, 1, 1, qivb);
+        SB(s, 1, 1, 2, qiwb); SB(s - 1, 1, 1, 2, qiwb);
         const real dwdzb = f43 * tzzb * oodz, dudxb = -f23 * tzzb * q4 * oodx, dvdyb = -f23 * tzzb * q4 * oodz;
         const real dudzb = txzb * oodz, dwdxb = txzb * q4 * oodx;
         const real dvdzb = tyzb * oodz, dwdyb = tyzb * q4 * oodz;
-        Sb[s][1][1][0] += dudzb; Sb[s - 1][1][1][0] -= dudzb;
-        Sb[s][1][1][1] += dvdzb; Sb[s - 1][1][1][1] -= dvdzb;
-        Sb[s][1][1][2] += dwdzb; Sb[s - 1][1][1][2] -= dwdzb;
-        Sb[s][1][2][0] += dudxb; Sb[1][s][0][0] -= dudxb; Sb[s - 1][1][2][0] += dudxb; Sb[1][s - 1][0][0] -= dudxb;
-        Sb[s][1][2][1] += dwdxb; Sb[1][s][0][2] -= dwdxb; Sb[s - 1][1][2][1] += dwdxb; Sb[1][s - 1][0][2] -= dwdxb;
-        Sb[s][2][1][2] += dvdyb; Sb[s][0][1][1] -= dvdyb; Sb[s - 1][2][1][2] += dvdyb; Sb[s - 1][0][1][1] -= dvdyb;
-        Sb[s][2][1][1] += dwdyb; Sb[s][0][1][2] -= dwdyb; Sb[s - 1][2][1][1] += dwdyb; Sb[s - 1][0][1][2] -= dwdyb;
+        SB(s, 1, 1, 0, dudzb); SB(s - 1, 1, 1, 0, -(dudzb));
+        SB(s, 1, 1, 1, dvdzb); SB(s - 1, 1, 1, 1, -(dvdzb));
+        SB(s, 1, 1, 2, dwdzb); SB(s - 1, 1, 1, 2, -(dwdzb));
+        SB(s, 1, 2, 0, dudxb); SB(1, s, 0, 0, -(dudxb)); SB(s - 1, 1, 2, 0, dudxb); SB(1, s - 1, 0, 0, -(dudxb));
+        SB(s, 1, 2, 1, dwdxb); SB(1, s, 0, 2, -(dwdxb)); SB(s - 1, 1, 2, 1, dwdxb); SB(1, s - 1, 0, 2, -(dwdxb));
+        SB(s, 2, 1, 2, dvdyb); SB(s, 0, 1, 1, -(dvdyb)); SB(s - 1, 2, 1, 2, dvdyb); SB(s - 1, 0, 1, 1, -(dvdyb));
+        SB(s, 2, 1, 1, dwdyb); SB(s, 0, 1, 2, -(dwdyb)); SB(s - 1, 2, 1, 1, dwdyb); SB(s - 1, 0, 1, 2, -(dwdyb));
       }
     }
   }
@@ -577,7 +577,9 @@ k_adj_diffusion(const real* __restrict__ U, const real* __restrict__ v, real sig
       real vb[4], out[4];
 #pragma unroll
       for (int c = 0; c < 4; ++c) vb[c] = sigma * v[t + (c + 1) * P.sf];
-      visc_cell<real, false>(S, oodx, oody, oodz, vb, out, Sb);
+      auto Sget = [&](int dk, int dj, int di, int c) -> real { return S[dk][dj][di][c]; };
+      auto Sacc = [&](int dk, int dj, int di, int c, real x) { Sb[dk][dj][di][c] += x; };
+      visc_cell<real, false>(Sget, oodx, oody, oodz, vb, out, Sacc);
       double gmu = 0.0;
 #pragma unroll
       for (int c = 0; c < 4; ++c) gmu += (double)vb[c] * (double)out[c];
@@ -586,7 +588,7 @@ k_adj_diffusion(const real* __restrict__ U, const real* __restrict__ v, real sig
       const real mdt = (real)P.mu * rdt;
 #pragma unroll
       for (int c = 0; c < 4; ++c) g[c] = vb[c] * mdt;
-      visc_cell<real, true>(S, oodx, oody, oodz, g, out, Sb);
+      visc_cell<real, true>(Sget, oodx, oody, oodz, g, out, Sacc);
 #pragma unroll
       for (int dk = 0; dk < 3; ++dk)
 #pragma unroll
@@ -601,6 +603,289 @@ k_adj_diffusion(const real* __restrict__ U, const real* __restrict__ v, real sig
     }
   }
   // parameter gradients: wave sums, then one atomic per wave
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    gk += __shfl_xor(gk, off);
+    gm += __shfl_xor(gm, off);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (TC && gk != 0.0) unsafeAtomicAdd(&grad[0], gk);
+    if (VISC && gm != 0.0) unsafeAtomicAdd(&grad[1], gm);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// LDS-tiled forms of the two scatter kernels above.  A workgroup stages the
+// primitives of its window (tile plus stencil halo) in LDS once, sums the
+// contributions of its faces or cells into an LDS accumulator of the same
+// window, and adds the window to Qb with one global atomic per value.
+// ---------------------------------------------------------------------------
+constexpr int AF_W = 64;  // faces (x) or lanes (y, z) per tile row
+constexpr int AF_R = 4;   // rows (x: y rows; y, z: face rows) per tile
+
+template <typename real, int DIR>
+struct FaceTile {
+  // window extents: DIR x: AF_R rows of AF_W+4 cells; y, z: AF_R+4 rows of AF_W
+  static constexpr int NWR = DIR == FV3D_IX ? AF_R : AF_R + 4;
+  static constexpr int NWC = DIR == FV3D_IX ? AF_W + 4 : AF_W;
+  static constexpr int NWIN = NWR * NWC;
+};
+
+template <typename real, int DIR, int RECON, int RS>
+__global__ void __launch_bounds__(256)
+k_adj_faces_tiled(const real* __restrict__ U, const real* __restrict__ v, real sigma, real* Qb,
+                  double dt, DevParams P) {
+  using T = FaceTile<real, DIR>;
+  __shared__ real sq[5][T::NWIN];
+  __shared__ real sa[5][T::NWIN];
+  const int tid = threadIdx.x;
+  const int Nd = DIR == FV3D_IX ? P.Nx : (DIR == FV3D_IY ? P.Ny : P.Nz);
+  const int bc = DIR == FV3D_IX ? P.bcx : (DIR == FV3D_IY ? P.bcy : P.bcz);
+  const long long stride = DIR == FV3D_IX ? 1 : (DIR == FV3D_IY ? P.sy : P.sz);
+  // tile origin: x0 (lanes or faces), d0 (first face along DIR), fixed coords
+  int i0, j0, k0;
+  if (DIR == FV3D_IX) { i0 = blockIdx.x * AF_W; j0 = blockIdx.y * AF_R; k0 = blockIdx.z; }
+  else if (DIR == FV3D_IY) { i0 = blockIdx.x * AF_W; j0 = blockIdx.y * AF_R; k0 = blockIdx.z; }
+  else { i0 = blockIdx.x * AF_W; j0 = blockIdx.y; k0 = blockIdx.z * AF_R; }
+  const int d0 = DIR == FV3D_IX ? i0 : (DIR == FV3D_IY ? j0 : k0);
+  // window cell w -> (coordinate along DIR, base offset of the other two)
+  auto win = [&](int w, int& g, bool& inside, long long& base) {
+    const int r = w / T::NWC, c = w % T::NWC;
+    int i, j, k;
+    if (DIR == FV3D_IX) { i = i0 - 2 + c; j = j0 + r; k = k0; g = i; inside = j < P.Ny; }
+    else if (DIR == FV3D_IY) { i = i0 + c; j = j0 - 2 + r; k = k0; g = j; inside = i < P.Nx; }
+    else { i = i0 + c; j = j0; k = k0 - 2 + r; g = k; inside = i < P.Nx; }
+    base = (DIR == FV3D_IX ? 0 : i) + (DIR == FV3D_IY ? 0 : (long long)j * P.sy) +
+           (DIR == FV3D_IZ ? 0 : (long long)k * P.sz);
+  };
+  const real gm1 = (real)P.gm1;
+  for (int w = tid; w < T::NWIN; w += blockDim.x) {
+    int g; bool inside; long long base;
+    win(w, g, inside, base);
+    real q[5] = {0, 0, 0, 0, 0};
+    if (inside) {
+      bool fl;
+      const int s = map_coord(g, Nd, bc, fl);
+      load_prim_arr(U, base + (long long)s * stride, P.sf, gm1, q);
+      if (fl) q[1 + DIR] = -q[1 + DIR];
+    }
+#pragma unroll
+    for (int n = 0; n < 5; ++n) { sq[n][w] = q[n]; sa[n][w] = 0; }
+  }
+  __syncthreads();
+
+  const double d = DIR == FV3D_IX ? P.dx : (DIR == FV3D_IY ? P.dy : P.dz);
+  const real coef = (real)((double)sigma * dt / d);
+  // face fm (relative to d0, -1..AF_*-1) in tile row/lane t: stencil window
+  // cells along DIR fm+1 .. fm+4
+  auto face = [&](int t, int fm) {
+    const int m = d0 + fm;
+    if (m < -1 || m > Nd - 1) return;
+    int wc[4];
+#pragma unroll
+    for (int o = 0; o < 4; ++o)
+      wc[o] = DIR == FV3D_IX ? t * T::NWC + (fm + 1 + o) : (fm + 1 + o) * T::NWC + t;
+    // transverse coordinate inside the domain?
+    {
+      int g; bool inside; long long base;
+      win(wc[1], g, inside, base);
+      if (!inside) return;
+      real q[4][5];
+#pragma unroll
+      for (int o = 0; o < 4; ++o)
+#pragma unroll
+        for (int n = 0; n < 5; ++n) q[o][n] = sq[n][wc[o]];
+      real Fb[5];
+      bool fl;
+      const long long cb = base + (long long)map_coord(m, Nd, bc, fl) * stride;
+      const long long cc = base + (long long)map_coord(m + 1, Nd, bc, fl) * stride;
+#pragma unroll
+      for (int n = 0; n < 5; ++n) {
+        const real vc = (m + 1 < Nd) ? v[cc + n * P.sf] : (real)0;
+        const real vb = (m >= 0) ? v[cb + n * P.sf] : (real)0;
+        Fb[n] = coef * (vc - vb);
+      }
+      real qL[5], qR[5], sb[5], sc[5];
+      int selb[5] = {0, 0, 0, 0, 0}, selc[5] = {0, 0, 0, 0, 0};
+      if (RECON == FV3D_PLM) {
+        slope_sel<real, DIR == FV3D_IZ>(q[0], q[1], q[2], sb, selb);
+        slope_sel<real, DIR == FV3D_IZ>(q[1], q[2], q[3], sc, selc);
+#pragma unroll
+        for (int n = 0; n < 5; ++n) {
+          qL[n] = q[1][n] + sb[n] * (real)0.5;
+          qR[n] = q[2][n] - sc[n] * (real)0.5;
+        }
+      } else {
+#pragma unroll
+        for (int n = 0; n < 5; ++n) { qL[n] = q[1][n]; qR[n] = q[2][n]; }
+      }
+      swap_dir<DIR>(qL);
+      swap_dir<DIR>(qR);
+      swap_dir<DIR>(Fb);
+      real Lb[5] = {0, 0, 0, 0, 0}, Rb[5] = {0, 0, 0, 0, 0};
+      if (RS == FV3D_HLL) hll_vjp(qL, qR, (real)P.gamma, gm1, Fb, Lb, Rb);
+      else hllc_vjp(qL, qR, (real)P.gamma, (real)P.entho, Fb, Lb, Rb);
+      swap_dir<DIR>(Lb);
+      swap_dir<DIR>(Rb);
+      real qb[4][5];
+#pragma unroll
+      for (int n = 0; n < 5; ++n) { qb[0][n] = 0; qb[1][n] = Lb[n]; qb[2][n] = Rb[n]; qb[3][n] = 0; }
+      if (RECON == FV3D_PLM) {
+        real ssb[5], ssc[5];
+#pragma unroll
+        for (int n = 0; n < 5; ++n) { ssb[n] = (real)0.5 * Lb[n]; ssc[n] = (real)-0.5 * Rb[n]; }
+        slope_vjp(selb, ssb, qb[0], qb[1], qb[2]);
+        slope_vjp(selc, ssc, qb[1], qb[2], qb[3]);
+      }
+#pragma unroll
+      for (int o = 0; o < 4; ++o)
+#pragma unroll
+        for (int n = 0; n < 5; ++n)
+          if (qb[o][n] != (real)0) unsafeAtomicAdd(&sa[n][wc[o]], qb[o][n]);
+    }
+  };
+  // AF_W x AF_R faces, one per thread; the tile at the low domain edge also
+  // owns face -1/2 of every row
+  if (DIR == FV3D_IX) {
+    face(tid / AF_W, tid % AF_W);
+    if (d0 == 0 && tid < AF_R) face(tid, -1);
+  } else {
+    face(tid % AF_W, tid / AF_W);
+    if (d0 == 0 && tid < AF_W) face(tid, -1);
+  }
+  __syncthreads();
+
+  for (int w = tid; w < T::NWIN; w += blockDim.x) {
+    int g; bool inside; long long base;
+    win(w, g, inside, base);
+    if (!inside || g < -2 || g > Nd + 1) continue;
+    bool fl;
+    const long long src = base + (long long)map_coord(g, Nd, bc, fl) * stride;
+#pragma unroll
+    for (int n = 0; n < 5; ++n) {
+      real x = sa[n][w];
+      if (fl && n == 1 + DIR) x = -x;
+      atomic_add(&Qb[src + n * P.sf], x);
+    }
+  }
+}
+
+constexpr int AD_X = 32, AD_Y = 4, AD_Z = 4;  // diffusion tile
+
+template <typename real, bool TC, bool VISC>
+__global__ void __launch_bounds__(AD_X * AD_Y * AD_Z)
+k_adj_diffusion_tiled(const real* __restrict__ U, const real* __restrict__ v, real sigma,
+                      real* Qb, double dt, DevParams P, double* grad) {
+  constexpr int WX = AD_X + 2, WY = AD_Y + 2, WZ = AD_Z + 2, NW = WX * WY * WZ;
+  __shared__ real sq[4][NW];  // u, v, w, T of the window (ghost-mapped)
+  __shared__ real sa[4][NW];  // their adjoints
+  const int tid = threadIdx.x;
+  const int i0 = blockIdx.x * AD_X, j0 = blockIdx.y * AD_Y, k0 = blockIdx.z * AD_Z;
+  const real gm1 = (real)P.gm1;
+  auto wcoord = [&](int w, int& i, int& j, int& k) {
+    i = i0 - 1 + w % WX;
+    j = j0 - 1 + (w / WX) % WY;
+    k = k0 - 1 + w / (WX * WY);
+  };
+  for (int w = tid; w < NW; w += blockDim.x) {
+    int i, j, k;
+    wcoord(w, i, j, k);
+    bool fx, fy, fz;
+    const int si = map_coord(i, P.Nx, P.bcx, fx), sj = map_coord(j, P.Ny, P.bcy, fy),
+              sk = map_coord(k, P.Nz, P.bcz, fz);
+    real q[5];
+    load_prim_arr(U, si + (long long)sj * P.sy + (long long)sk * P.sz, P.sf, gm1, q);
+    sq[0][w] = fx ? -q[1] : q[1];
+    sq[1][w] = fy ? -q[2] : q[2];
+    sq[2][w] = fz ? -q[3] : q[3];
+    sq[3][w] = q[4] / q[0];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) sa[c][w] = 0;
+  }
+  __syncthreads();
+
+  const int ti = tid % AD_X, tj = (tid / AD_X) % AD_Y, tk = tid / (AD_X * AD_Y);
+  const int i = i0 + ti, j = j0 + tj, k = k0 + tk;
+  double gk = 0.0, gm = 0.0;
+  if (i < P.Nx && j < P.Ny && k < P.Nz) {
+    const long long cell = i + (long long)j * P.sy + (long long)k * P.sz;
+    auto W = [&](int dk, int dj, int di) {  // window index of offset (di-1, dj-1, dk-1)
+      return (ti + di) + WX * ((tj + dj) + WY * (tk + dk));
+    };
+    const real rdt = (real)dt;
+    if (TC) {
+      const real Eb = sigma * v[cell + 4 * P.sf];
+      const int nb[7] = {W(1, 1, 1), W(1, 1, 0), W(1, 1, 2), W(1, 0, 1), W(1, 2, 1), W(0, 1, 1), W(2, 1, 1)};
+      const real ax = rdt / (real)(P.dx * P.dx), ay = rdt / (real)(P.dy * P.dy),
+                 az = rdt / (real)(P.dz * P.dz);
+      real c[7] = {-(real)2 * (ax + ay + az), ax, ax, ay, ay, az, az};
+      real fixedE = 0;
+      if (k == 0 && P.bctc_zmin != FV3D_BCTC_NONE) {
+        c[0] += az; c[5] -= az;
+        if (P.bctc_zmin == FV3D_BCTC_FIXED_TEMPERATURE) {
+          c[0] -= (real)2 * az;
+          fixedE += (real)2 * az * (real)P.bctc_zmin_value;
+        } else if (P.bctc_zmin == FV3D_BCTC_FIXED_GRADIENT) {
+          fixedE -= rdt / (real)P.dz * (real)P.bctc_zmin_value;
+        }
+      }
+      if (k == P.Nz - 1 && P.bctc_zmax != FV3D_BCTC_NONE) {
+        c[0] += az; c[6] -= az;
+        if (P.bctc_zmax == FV3D_BCTC_FIXED_TEMPERATURE) {
+          c[0] -= (real)2 * az;
+          fixedE += (real)2 * az * (real)P.bctc_zmax_value;
+        } else if (P.bctc_zmax == FV3D_BCTC_FIXED_GRADIENT) {
+          fixedE += rdt / (real)P.dz * (real)P.bctc_zmax_value;
+        }
+      }
+      real dE1 = fixedE;
+#pragma unroll
+      for (int o = 0; o < 7; ++o) dE1 += c[o] * sq[3][nb[o]];
+      gk = (double)Eb * (double)dE1;
+      const real kf = (real)P.kface;
+#pragma unroll
+      for (int o = 0; o < 7; ++o) {
+        const real x = Eb * kf * c[o];
+        if (x != (real)0) unsafeAtomicAdd(&sa[3][nb[o]], x);
+      }
+    }
+    if (VISC) {
+      const real oodx = (real)1 / (real)P.dx, oody = (real)1 / (real)P.dy, oodz = (real)1 / (real)P.dz;
+      real vb[4], out[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) vb[c] = sigma * v[cell + (c + 1) * P.sf];
+      auto Sget = [&](int dk, int dj, int di, int c) -> real { return sq[c][W(dk, dj, di)]; };
+      auto Sacc = [&](int dk, int dj, int di, int c, real x) {
+        if (x != (real)0) unsafeAtomicAdd(&sa[c][W(dk, dj, di)], x);
+      };
+      visc_cell<real, false>(Sget, oodx, oody, oodz, vb, out, Sacc);
+      double gmu = 0.0;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) gmu += (double)vb[c] * (double)out[c];
+      gm = gmu * dt;
+      real g[4];
+      const real mdt = (real)P.mu * rdt;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) g[c] = vb[c] * mdt;
+      visc_cell<real, true>(Sget, oodx, oody, oodz, g, out, Sacc);
+    }
+  }
+  __syncthreads();
+  for (int w = tid; w < NW; w += blockDim.x) {
+    int wi, wj, wk;
+    wcoord(w, wi, wj, wk);
+    if (wi < -1 || wi > P.Nx || wj < -1 || wj > P.Ny || wk < -1 || wk > P.Nz) continue;
+    bool fx, fy, fz;
+    const int si = map_coord(wi, P.Nx, P.bcx, fx), sj = map_coord(wj, P.Ny, P.bcy, fy),
+              sk = map_coord(wk, P.Nz, P.bcz, fz);
+    const long long src = si + (long long)sj * P.sy + (long long)sk * P.sz;
+    if (VISC) {
+      atomic_add(&Qb[src + 1 * P.sf], fx ? -sa[0][w] : sa[0][w]);
+      atomic_add(&Qb[src + 2 * P.sf], fy ? -sa[1][w] : sa[1][w]);
+      atomic_add(&Qb[src + 3 * P.sf], fz ? -sa[2][w] : sa[2][w]);
+    }
+    if (TC) atomic_add(&Qb[src + 5 * P.sf], sa[3][w]);
+  }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
     gk += __shfl_xor(gk, off);
@@ -652,11 +937,20 @@ k_adj_finalize(const real* __restrict__ U, const real* __restrict__ v, real sigm
 template <typename real, int DIR, int RECON, int RS>
 static hipError_t adj_faces_dir(const AdjLaunch<real>& L, hipStream_t s) {
   const DevParams& P = L.P;
-  const long long nf = (long long)(P.Nx + (DIR == FV3D_IX)) * (P.Ny + (DIR == FV3D_IY)) *
-                       (P.Nz + (DIR == FV3D_IZ));
-  const int bs = 256;
-  hipLaunchKernelGGL((k_adj_faces<real, DIR, RECON, RS>), dim3((unsigned)((nf + bs - 1) / bs)),
-                     dim3(bs), 0, s, L.U, L.v, L.sigma, L.Qb, L.dt, P);
+  if (L.simple) {
+    const long long nf = (long long)(P.Nx + (DIR == FV3D_IX)) * (P.Ny + (DIR == FV3D_IY)) *
+                         (P.Nz + (DIR == FV3D_IZ));
+    const int bs = 256;
+    hipLaunchKernelGGL((k_adj_faces<real, DIR, RECON, RS>), dim3((unsigned)((nf + bs - 1) / bs)),
+                       dim3(bs), 0, s, L.U, L.v, L.sigma, L.Qb, L.dt, P);
+    return hipGetLastError();
+  }
+  dim3 grid;
+  if (DIR == FV3D_IX) grid = dim3((P.Nx + AF_W - 1) / AF_W, (P.Ny + AF_R - 1) / AF_R, P.Nz);
+  else if (DIR == FV3D_IY) grid = dim3((P.Nx + AF_W - 1) / AF_W, (P.Ny + AF_R - 1) / AF_R, P.Nz);
+  else grid = dim3((P.Nx + AF_W - 1) / AF_W, P.Ny, (P.Nz + AF_R - 1) / AF_R);
+  hipLaunchKernelGGL((k_adj_faces_tiled<real, DIR, RECON, RS>), grid, dim3(AF_W * AF_R), 0, s, L.U,
+                     L.v, L.sigma, L.Qb, L.dt, P);
   return hipGetLastError();
 }
 
@@ -666,6 +960,20 @@ static hipError_t adj_faces(const AdjLaunch<real>& L, hipStream_t s) {
   if (e == hipSuccess) e = adj_faces_dir<real, FV3D_IY, RECON, RS>(L, s);
   if (e == hipSuccess) e = adj_faces_dir<real, FV3D_IZ, RECON, RS>(L, s);
   return e;
+}
+
+template <typename real, bool TC, bool VISC>
+static void adj_diffusion(const AdjLaunch<real>& L, hipStream_t s) {
+  const DevParams& P = L.P;
+  if (L.simple) {
+    const int bs = 256;
+    hipLaunchKernelGGL((k_adj_diffusion<real, TC, VISC>), dim3((unsigned)((P.sf + bs - 1) / bs)),
+                       dim3(bs), 0, s, L.U, L.v, L.sigma, L.Qb, L.dt, P, L.grad);
+    return;
+  }
+  const dim3 grid((P.Nx + AD_X - 1) / AD_X, (P.Ny + AD_Y - 1) / AD_Y, (P.Nz + AD_Z - 1) / AD_Z);
+  hipLaunchKernelGGL((k_adj_diffusion_tiled<real, TC, VISC>), grid, dim3(AD_X * AD_Y * AD_Z), 0, s,
+                     L.U, L.v, L.sigma, L.Qb, L.dt, P, L.grad);
 }
 
 // one adjoint stage: Qb must be zero on entry and is zero on return
@@ -678,19 +986,13 @@ static hipError_t launch_adjoint_stage_t(const AdjLaunch<real>& L, hipStream_t s
   else if (hll) e = adj_faces<real, FV3D_PCM, FV3D_HLL>(L, s);
   else e = adj_faces<real, FV3D_PCM, FV3D_HLLC>(L, s);
   if (e != hipSuccess) return e;
-  const int bs = 256;
-  const unsigned nb = (unsigned)((L.P.sf + bs - 1) / bs);
-  if (L.tc && L.visc)
-    hipLaunchKernelGGL((k_adj_diffusion<real, true, true>), dim3(nb), dim3(bs), 0, s, L.U, L.v,
-                       L.sigma, L.Qb, L.dt, L.P, L.grad);
-  else if (L.tc)
-    hipLaunchKernelGGL((k_adj_diffusion<real, true, false>), dim3(nb), dim3(bs), 0, s, L.U, L.v,
-                       L.sigma, L.Qb, L.dt, L.P, L.grad);
-  else if (L.visc)
-    hipLaunchKernelGGL((k_adj_diffusion<real, false, true>), dim3(nb), dim3(bs), 0, s, L.U, L.v,
-                       L.sigma, L.Qb, L.dt, L.P, L.grad);
+  if (L.tc && L.visc) adj_diffusion<real, true, true>(L, s);
+  else if (L.tc) adj_diffusion<real, true, false>(L, s);
+  else if (L.visc) adj_diffusion<real, false, true>(L, s);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
+  const int bs = 256;
+  const unsigned nb = (unsigned)((L.P.sf + bs - 1) / bs);
   hipLaunchKernelGGL((k_adj_finalize<real>), dim3(nb), dim3(bs), 0, s, L.U, L.v, L.sigma,
                      L.extra, L.alpha, L.Qb, L.out, L.P);
   return hipGetLastError();

@@ -83,7 +83,8 @@ inline DevParams make_devparams(const fv3d_params& p, int nzc_override = 0) {
   d.inv_tc = inv_dt_par(p.thermal_conductivity_active != 0, p.kappa, p);
   d.inv_visc = inv_dt_par(p.viscosity_active != 0, p.mu, p);
   d.tiles_x = (p.Nx + STAGE_WX - 1) / STAGE_WX;
-  d.tiles_y = (p.Ny + STAGE_TY - 1) / STAGE_TY;
+  const int ty = p.viscosity_active ? STAGE_TY_VISC : STAGE_TY;
+  d.tiles_y = (p.Ny + ty - 1) / ty;
   d.nzc = nzc_override > 0 ? std::max(1, std::min(nzc_override, p.Nz)) : tiles_z_for(p, d.tiles_x * d.tiles_y);
   d.tiles_z = (p.Nz + d.nzc - 1) / d.nzc;
   return d;

@@ -8,6 +8,9 @@ namespace fv3d_hip {
 
 // rows per stage tile (the tile is 64 x STAGE_TY columns)
 constexpr int STAGE_TY = 8;
+// with viscosity: 4 rows (8 waves), which leaves 256 VGPRs per lane for the
+// viscous stencil instead of spilling at 168
+constexpr int STAGE_TY_VISC = 4;
 constexpr int STAGE_WX = 64;
 
 template <typename real>
@@ -80,6 +83,7 @@ struct AdjLaunch {
   DevParams P;
   int recon, riemann;
   bool tc, visc;
+  bool simple = false;  // one thread per face / cell with global atomics (check path)
 };
 template <typename real>
 hipError_t launch_adjoint_stage(const AdjLaunch<real>& L, hipStream_t s);

@@ -600,9 +600,9 @@ static hipError_t launch_t(const StageLaunch<real>& L, hipStream_t s) {
 template <typename real, int RECON, int RS, int MODE>
 static hipError_t launch_phys(const StageLaunch<real>& L, hipStream_t s) {
   const bool tc = L.tc, vi = L.visc;
-  if (tc && vi) return launch_t<real, STAGE_TY, RECON, RS, true, true, MODE>(L, s);
+  if (tc && vi) return launch_t<real, STAGE_TY_VISC, RECON, RS, true, true, MODE>(L, s);
   if (tc) return launch_t<real, STAGE_TY, RECON, RS, true, false, MODE>(L, s);
-  if (vi) return launch_t<real, STAGE_TY, RECON, RS, false, true, MODE>(L, s);
+  if (vi) return launch_t<real, STAGE_TY_VISC, RECON, RS, false, true, MODE>(L, s);
   return launch_t<real, STAGE_TY, RECON, RS, false, false, MODE>(L, s);
 }
 

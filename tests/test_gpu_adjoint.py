@@ -138,6 +138,19 @@ def test_adjoint_checkpoint_interval_does_not_change_result(gpu_available):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("name", list(CONFIGS))
+def test_tiled_adjoint_equals_per_face_form(gpu_available, name, monkeypatch):
+    """The LDS-tiled kernels against the one-thread-per-face/cell kernels
+    (FV3D_ADJ_SIMPLE): the same sums, associated differently."""
+    pr = Problem(base_params(N=20, **CONFIGS[name]), nsteps=5)
+    _, gt, kt, mt = pr.adjoint(K=2)
+    monkeypatch.setenv("FV3D_ADJ_SIMPLE", "1")
+    _, gs, ks, ms = pr.adjoint(K=2)
+    assert np.abs(gt - gs).max() <= 1e-12 * np.abs(gs).max()
+    assert kt == pytest.approx(ks, rel=1e-12, abs=1e-300) and mt == pytest.approx(ms, rel=1e-12, abs=1e-300)
+
+
+@pytest.mark.gpu
 def test_adjoint_fp32_against_fp64(gpu_available):
     pr = Problem(base_params(N=24), nsteps=12)
     _, g64, k64, m64 = pr.adjoint(precision=F.FP64)
