@@ -214,6 +214,9 @@ int adjoint_run_t(fv3d_ctx* c, int nsteps, int K, const double* dts_dev,
 int check_adjoint_params(fv3d_ctx* c) {
   const fv3d_params& p = c->p;
   if (p.gravity) return ctx_fail(c, FV3D_EINVAL, "adjoint: gravity is not supported");
+  if (p.boundary_z == FV3D_BC_C91) return ctx_fail(c, FV3D_EINVAL, "adjoint: boundaries_z=C91 is not supported");
+  if (p.thermal_conductivity_active && p.thermal_conductivity_mode == FV3D_TCM_B02)
+    return ctx_fail(c, FV3D_EINVAL, "adjoint: conductivity_mode=B02 is not supported");
   if (p.reconstruction == FV3D_PCM_WB || p.well_balanced_flux_at_z_bc)
     return ctx_fail(c, FV3D_EINVAL, "adjoint: well-balanced variants are not supported");
   return FV3D_OK;

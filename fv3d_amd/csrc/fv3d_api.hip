@@ -84,6 +84,19 @@ int fv3d_create(const fv3d_params* p, int precision, int device, fv3d_ctx** out)
   if (kahan(c) && (hipMalloc(&c->C, c->state_bytes) != hipSuccess ||
                    hipMalloc(&c->Cmp, c->state_bytes) != hipSuccess))
     return cleanup(FV3D_ENOMEM, "state allocation failed");
+  if (p->boundary_z == FV3D_BC_C91) {
+    if (hipMalloc(&c->c91g, 4 * 5 * (size_t)p->Nx * p->Ny * c->elem) != hipSuccess)
+      return cleanup(FV3D_ENOMEM, "allocation failed");
+    c->dp.c91g = c->c91g;
+  }
+  if (p->thermal_conductivity_active && p->thermal_conductivity_mode == FV3D_TCM_B02) {
+    const std::vector<double> t = b02_face_table(*p);
+    if (hipMalloc(&c->kztab, t.size() * sizeof(double)) != hipSuccess)
+      return cleanup(FV3D_ENOMEM, "allocation failed");
+    if (hipMemcpy(c->kztab, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+      return cleanup(FV3D_EHIP, "copy failed");
+    c->dp.kztab = c->kztab;
+  }
   if (hipMalloc(&c->ring, 4 * sizeof(unsigned long long)) != hipSuccess ||
       hipMalloc(&c->health, sizeof(int)) != hipSuccess)
     return cleanup(FV3D_ENOMEM, "allocation failed");
@@ -105,6 +118,8 @@ int fv3d_destroy(fv3d_ctx* c) {
   if (c->ring) hipFree(c->ring);
   if (c->health) hipFree(c->health);
   if (c->qbuf) hipFree(c->qbuf);
+  if (c->c91g) hipFree(c->c91g);
+  if (c->kztab) hipFree(c->kztab);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return FV3D_OK;

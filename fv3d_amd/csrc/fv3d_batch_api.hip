@@ -158,6 +158,10 @@ int fv3d_batch_create(int nbox, const fv3d_params* params, int precision, int de
     std::string why;
     if (check_params(params[i], why)) return record_error(FV3D_EINVAL, "box " + std::to_string(i) + ": " + why);
     const fv3d_params& p = params[i];
+    if (p.boundary_z == FV3D_BC_C91)
+      return record_error(FV3D_EINVAL, "box " + std::to_string(i) + ": boundaries_z=C91 is not supported in ensembles");
+    if (p.thermal_conductivity_active && p.thermal_conductivity_mode == FV3D_TCM_B02)
+      return record_error(FV3D_EINVAL, "box " + std::to_string(i) + ": conductivity_mode=B02 is not supported in ensembles");
     if (p.reconstruction != p0.reconstruction || p.riemann_solver != p0.riemann_solver ||
         p.time_stepping != p0.time_stepping ||
         p.thermal_conductivity_active != p0.thermal_conductivity_active ||

@@ -24,6 +24,8 @@ struct fv3d_ctx {
   unsigned long long* ring = nullptr;  // 3 slots
   int* health = nullptr;
   double* qbuf = nullptr;  // ghosted SoA primitive staging, allocated on first use
+  void* c91g = nullptr;    // boundary_z == C91: z-ghost planes of the stage input
+  double* kztab = nullptr; // conductivity_mode == B02: face conductivities per z level
   long step = 0;
   bool have_state = false;
   bool have_stage = false;  // B holds the reference's post-update Q source

@@ -33,6 +33,13 @@ struct DevParams {
   double CFL, inv_tc, inv_visc; // dt inputs; inv_tc/inv_visc are constants of the run
   int tiles_x, tiles_y, tiles_z, nzc;
   int dt_direct;                // the dt slot holds dt itself (a frozen dt sequence)
+  // boundary_z == BC_C91: primitives of the four z-ghost planes (k = -2, -1,
+  // Nz, Nz+1) as [plane][5][Ny][Nx], written before each stage
+  void* c91g;
+  double zmin, theta1;
+  // conductivity_mode == B02: face conductivities per z level, [3][Nz] =
+  // (x and y faces, lower z face, upper z face), from the host
+  const double* kztab;
 };
 
 template <typename real>

@@ -42,14 +42,36 @@ inline int tiles_z_for(const fv3d_params& p, int tiles_xy) {
   return std::max(1, std::min(nzc, p.Nz));
 }
 
+// computeKappa (ThermalConduction.h:7-22), evaluated on the host with the
+// host's tanh as the reference's OpenMP build does
+inline double compute_kappa(const fv3d_params& p, double z) {
+  if (p.thermal_conductivity_mode == FV3D_TCM_B02) {
+    const double tr = (std::tanh((z - p.b02_zmid) / p.b02_thickness) + 1.0) * 0.5;
+    return p.kappa * (p.b02_kappa1 * (1.0 - tr) + p.b02_kappa2 * tr);
+  }
+  return p.kappa;
+}
+
+// B02 face conductivities per z level, [3][Nz]: the x/y faces (kappaL..D),
+// the lower and the upper z face (kappaF, kappaB), ThermalConduction.h:44-49
+inline std::vector<double> b02_face_table(const fv3d_params& p) {
+  std::vector<double> t(3 * (size_t)p.Nz);
+  for (int k = 0; k < p.Nz; ++k) {
+    const double z = p.zmin + (k + 0.5) * p.dz;  // getPos, SimInfo.h:162-167
+    const double kc = compute_kappa(p, z);
+    t[k] = 0.5 * (kc + compute_kappa(p, z));
+    t[p.Nz + k] = 0.5 * (kc + compute_kappa(p, z - p.dz));
+    t[2 * p.Nz + k] = 0.5 * (kc + compute_kappa(p, z + p.dz));
+  }
+  return t;
+}
+
 inline int check_params(const fv3d_params& p, std::string& why) {
   if (p.Nx < 2 || p.Ny < 2 || p.Nz < 2) { why = "Nx, Ny, Nz must be >= 2"; return 1; }
   if (p.Ng < (p.reconstruction == FV3D_PLM ? 2 : 1)) { why = "too few ghost layers"; return 1; }
-  if (p.boundary_z == FV3D_BC_C91) { why = "boundaries_z=C91 is not supported by the fused stage"; return 1; }
-  if (p.thermal_conductivity_active && p.thermal_conductivity_mode != FV3D_TCM_CONSTANT) {
-    why = "conductivity_mode=B02 is not supported";
-    return 1;
-  }
+  if (p.thermal_conductivity_active && p.thermal_conductivity_mode == FV3D_TCM_B02 &&
+      !(p.b02_thickness != 0.0))
+    { why = "conductivity_mode=B02 needs a non-zero b02 thickness"; return 1; }
   if (p.riemann_solver != FV3D_HLL && p.riemann_solver != FV3D_HLLC) { why = "riemann_solver"; return 1; }
   if (p.reconstruction < FV3D_PCM || p.reconstruction > FV3D_PLM) { why = "reconstruction"; return 1; }
   if (!(p.dx > 0 && p.dy > 0 && p.dz > 0)) { why = "non-positive cell size"; return 1; }
@@ -81,7 +103,22 @@ inline DevParams make_devparams(const fv3d_params& p, int nzc_override = 0) {
   d.mu = p.mu;
   d.CFL = p.CFL;
   d.inv_tc = inv_dt_par(p.thermal_conductivity_active != 0, p.kappa, p);
+  if (p.thermal_conductivity_active && p.thermal_conductivity_mode == FV3D_TCM_B02) {
+    // ComputeDt.h:39-41 passes the ghosted integer indices as coordinates
+    double m = 0.0;
+    for (int kg = p.kbeg; kg < p.kend; ++kg) {
+      const double kap = compute_kappa(p, (double)kg);
+      const double v = std::fmax(std::fmax(2.0 * kap / (p.dx * p.dx), 2.0 * kap / (p.dy * p.dy)),
+                                 2.0 * kap / (p.dz * p.dz));
+      if (kg == p.kbeg || v > m) m = v;
+    }
+    d.inv_tc = m;
+  }
+  d.kztab = nullptr;  // set by the owner of the table
   d.inv_visc = inv_dt_par(p.viscosity_active != 0, p.mu, p);
+  d.zmin = p.zmin;
+  d.theta1 = p.theta1;
+  d.c91g = nullptr;  // set by the owner of the ghost-plane buffer
   d.tiles_x = (p.Nx + STAGE_WX - 1) / STAGE_WX;
   const int ty = p.viscosity_active ? STAGE_TY_VISC : STAGE_TY;
   d.tiles_y = (p.Ny + ty - 1) / ty;

@@ -181,12 +181,23 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
   const real dt = (real)dtd;
   if (inv_zero && tile == 0 && threadIdx.x == 0) *inv_zero = 0ull;
 
+  // C91 z ghosts come precomputed (fillC91, BoundaryConditions.h:86-134):
+  // the column's value with the x/y reflections applied
+  auto c91_ghost = [&](int kk, long long xy, bool gx, bool gy) -> Prim<real> {
+    const real* g = (const real*)P.c91g + (long long)(kk < 0 ? kk + 2 : kk - P.Nz + 2) * 5 * P.sz + xy;
+    Prim<real> q{g[0], g[P.sz], g[2 * P.sz], g[3 * P.sz], g[4 * P.sz]};
+    if (gx) q.u = -q.u;
+    if (gy) q.v = -q.v;
+    return q;
+  };
   auto load_plane = [&](int kk) -> Prim<real> {
+    if (P.bcz == FV3D_BC_C91 && (kk < 0 || kk >= P.Nz)) return c91_ghost(kk, sxy, fx, fy);
     bool fz;
     const int sk = map_coord(kk, P.Nz, P.bcz, fz);
     return load_prim<real, MODE>(Uin, Kin, sxy + (long long)sk * P.sz, P.sf, gm1, fx, fy, fz);
   };
   auto load_halo = [&](int kk) -> Prim<real> {
+    if (P.bcz == FV3D_BC_C91 && (kk < 0 || kk >= P.Nz)) return c91_ghost(kk, hxy, hfx, hfy);
     bool fz;
     const int sk = map_coord(kk, P.Nz, P.bcz, fz);
     return load_prim<real, MODE>(Uin, Kin, hxy + (long long)sk * P.sz, P.sf, gm1, hfx, hfy, fz);
@@ -393,23 +404,30 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
         auto ddx = [&](real v) { return P.pow2x ? v * (real)P.idx : v / (real)P.dx; };
         auto ddy = [&](real v) { return P.pow2y ? v * (real)P.idy : v / (real)P.dy; };
         auto ddz = [&](real v) { return P.pow2z ? v * (real)P.idz : v / (real)P.dz; };
-        const real FL = ddx(kf * (TC_ - TL));
-        const real FR = ddx(kf * (TR - TC_));
-        const real FU = ddy(kf * (TC_ - TU));
-        const real FD = ddy(kf * (TD - TC_));
-        real FF = ddz(kf * (TC_ - TF));
-        real FB = ddz(kf * (TB - TC_));
+        // face conductivities: constant, or B02's z profile (kappaL..kappaB)
+        real kxy = kf, kzF = kf, kzB = kf;
+        if (P.kztab) {
+          kxy = (real)P.kztab[k];
+          kzF = (real)P.kztab[P.Nz + k];
+          kzB = (real)P.kztab[2 * P.Nz + k];
+        }
+        const real FL = ddx(kxy * (TC_ - TL));
+        const real FR = ddx(kxy * (TR - TC_));
+        const real FU = ddy(kxy * (TC_ - TU));
+        const real FD = ddy(kxy * (TD - TC_));
+        real FF = ddz(kzF * (TC_ - TF));
+        real FB = ddz(kzB * (TB - TC_));
         if (k == 0 && P.bctc_zmin != FV3D_BCTC_NONE) {
           if (P.bctc_zmin == FV3D_BCTC_FIXED_TEMPERATURE)
-            FF = ddz(kf * (real)2.0 * (TC_ - (real)P.bctc_zmin_value));
+            FF = ddz(kzF * (real)2.0 * (TC_ - (real)P.bctc_zmin_value));
           else if (P.bctc_zmin == FV3D_BCTC_FIXED_GRADIENT)
-            FF = kf * (real)P.bctc_zmin_value;
+            FF = kzF * (real)P.bctc_zmin_value;
         }
         if (k == P.Nz - 1 && P.bctc_zmax != FV3D_BCTC_NONE) {
           if (P.bctc_zmax == FV3D_BCTC_FIXED_TEMPERATURE)
-            FB = ddz(kf * (real)2.0 * ((real)P.bctc_zmax_value - TC_));
+            FB = ddz(kzB * (real)2.0 * ((real)P.bctc_zmax_value - TC_));
           else if (P.bctc_zmax == FV3D_BCTC_FIXED_GRADIENT)
-            FB = kf * (real)P.bctc_zmax_value;
+            FB = kzB * (real)P.bctc_zmax_value;
         }
         u[4] = u[4] + (dtdx * (FR - FL) + dtdy * (FD - FU) + dtdz * (FB - FF));
       }
@@ -606,8 +624,64 @@ static hipError_t launch_phys(const StageLaunch<real>& L, hipStream_t s) {
   return launch_t<real, STAGE_TY, RECON, RS, false, false, MODE>(L, s);
 }
 
+// fillC91 (BoundaryConditions.h:86-134) for the z-ghost planes of every
+// domain column, from the stage's input state, as primitives.  The
+// reference multiplies by the gravity flag where g belongs (its Q17); so
+// does this.
+template <typename real, int MODE>
+__global__ void __launch_bounds__(256)
+k_c91_ghosts(const real* __restrict__ U, const real* __restrict__ K, real* g, DevParams P) {
+  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (t >= 4 * P.sz) return;
+  const int plane = (int)(t / P.sz);
+  const long long xy = t % P.sz;
+  const int kk = plane < 2 ? plane - 2 : P.Nz + plane - 2;
+  const int kref = kk < 0 ? 0 : P.Nz - 1;
+  const int ksym = kk < 0 ? -kk - 1 : 2 * P.Nz - kk - 1;
+  const real gm1 = (real)P.gm1;
+  const Prim<real> qr = load_prim<real, MODE>(U, K, xy + (long long)kref * P.sz, P.sf, gm1, false, false, false);
+  const Prim<real> qs = load_prim<real, MODE>(U, K, xy + (long long)ksym * P.sz, P.sf, gm1, false, false, false);
+  const real z = (real)P.zmin + ((real)kk + (real)0.5) * (real)P.dz;
+  const real grav = P.gravity ? (real)1 : (real)0;
+  real res[5];
+  if (kk < 0) {
+    const real dz = z - (real)0.5 * (real)P.dz;
+    const real T = (real)1.0 + z * (real)P.theta1;
+    res[4] = qr.p + dz * qr.r * grav;
+    res[0] = res[4] / T;
+    res[4] = fmax(res[4], (real)1.0e-6);
+    res[0] = fmax(res[0], (real)1.0e-6);
+    const real dratio = qs.r / res[0];
+    res[1] = qs.u;
+    res[2] = qs.v;
+    res[3] = -qs.w * dratio;
+  } else {
+    const real Tref = qr.p / qr.r;
+    const real dz = z - ((real)1.0 - (real)0.5 * (real)P.dz);
+    const real T = Tref + dz * (real)P.theta1;
+    const real Pp = qr.p + dz * qr.r * grav;
+    const real rho = Pp / T;
+    res[4] = fmax(Pp, (real)1.0e-6);
+    res[0] = fmax(rho, (real)1.0e-6);
+    const real dratio = qs.r / rho;
+    res[1] = qs.u * dratio;
+    res[2] = qs.v * dratio;
+    res[3] = -qs.w * dratio;
+  }
+  real* d = g + (long long)plane * 5 * P.sz + xy;
+#pragma unroll
+  for (int n = 0; n < 5; ++n) d[n * P.sz] = res[n];
+}
+
 template <typename real, int MODE>
 static hipError_t launch_mode(const StageLaunch<real>& L, hipStream_t s) {
+  if (L.P.bcz == FV3D_BC_C91) {
+    const long long n = 4 * L.P.sz;
+    hipLaunchKernelGGL((k_c91_ghosts<real, MODE>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                       L.Uin, L.Kin, (real*)L.P.c91g, L.P);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
   const bool plm = L.recon == FV3D_PLM;
   const bool hll = L.riemann == FV3D_HLL;
   if (plm && hll) return launch_phys<real, FV3D_PLM, FV3D_HLL, MODE>(L, s);

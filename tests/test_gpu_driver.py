@@ -21,6 +21,9 @@ DRIVER = os.path.join(ROOT, "fv3d_amd", "fv3d")
 SOD_INI = os.path.join(ROOT, "tests", "data", "sod_c1.ini")
 
 
+C91_INI = os.path.join(ROOT, "tests", "data", "c91_small.ini")
+
+
 def _run_driver(ini, cwd, *extra):
     return subprocess.run([DRIVER, ini, *extra], cwd=cwd, capture_output=True, text=True, timeout=300)
 
@@ -87,3 +90,26 @@ def test_driver_fp32_runs(gpu_available, tmp_path):
     with H5File(str(tmp_path / "sod.h5")) as f:
         rho = f.dataset("ite_10/rho")
     assert np.all(np.isfinite(rho)) and rho.min() > 0.1
+
+
+@pytest.mark.gpu
+def test_driver_c91_settings_match_oracle(gpu_available, tmp_path):
+    """The reference's settings/C91.ini setup (C91 z boundaries, conduction
+    with fixed-temperature / fixed-gradient walls, viscosity, gravity) on a
+    16^3 grid: every snapshot of the driver's file against the oracle."""
+    shutil.copy(C91_INI, tmp_path / "c91.ini")
+    r = _run_driver("c91.ini", str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    cfg = read_inifile(C91_INI)
+    p = cfg.params
+    Q0 = F.problems.initial_condition(cfg)
+    steps, snaps = MainLoop(p, Q0).run()
+    assert len(snaps) >= 3
+    # settings/C91.ini names no output file: the default name applies
+    with H5File(str(tmp_path / (cfg.output_filename + ".h5"))) as f:
+        for ite, t, Q in snaps:
+            g = f"ite_{ite}"
+            assert f.group_double_attr(g, "time") == t
+            d = Q[p.kbeg:p.kend, p.jbeg:p.jend, p.ibeg:p.iend]
+            for field, slot in [("rho", 0), ("u", 1), ("v", 2), ("w", 3), ("prs", 4)]:
+                assert np.array_equal(f.dataset(f"{g}/{field}"), d[..., slot]), (g, field)

@@ -185,3 +185,51 @@ def test_sod_axes_transposed(gpu_available):
     qy = dom(py, ny[-1][2])
     qy_t = np.transpose(qy, (0, 2, 1, 3))[..., [0, 2, 1, 3, 4]]
     _assert_same(qx, qy_t, "sod_x vs transposed sod_y")
+
+
+def c91_params(**kw):
+    """settings/C91.ini of the reference (convection under fillC91 z
+    boundaries, conduction with a fixed-temperature bottom and
+    fixed-gradient top, viscosity, gravity g=40), on a small grid."""
+    base = dict(Nx=16, Ny=16, Nz=16, xmax=4.0, ymax=4.0, boundary_x=F.BC_PERIODIC,
+                boundary_y=F.BC_PERIODIC, boundary_z=F.BC_C91, riemann_solver=F.HLLC,
+                reconstruction=F.PLM, CFL=0.1, gamma0=1.66666667, gravity=1, g=40.0, c91_pert=1.0e-3,
+                theta1=20.0, m1=1.0, thermal_conductivity_active=1, kappa=0.07,
+                bctc_zmin=F.BCTC_FIXED_TEMPERATURE, bctc_zmin_value=1.0,
+                bctc_zmax=F.BCTC_FIXED_GRADIENT, bctc_zmax_value=20.0,
+                viscosity_active=1, mu=0.0028, tend=200.0)
+    base.update(kw)
+    return Params.defaults().replace(**base)
+
+
+@pytest.mark.parametrize("ts", [F.TS_EULER, F.TS_RK2])
+def test_c91_z_boundaries_bitwise(gpu_available, ts):
+    """boundaries_z=C91 in the fused stage (ghost planes from fillC91,
+    BoundaryConditions.h:86-134, and the C91 z-wall flux of
+    Update.h:138-143) against the oracle, Euler as in C91.ini and RK2."""
+    p = c91_params(time_stepping=ts)
+    _compare_runs(p, F.problems.c91(p), max_steps=8)
+
+
+def test_c91_z_with_reflecting_x_hll_pcm(gpu_available):
+    """C91 z ghosts of reflected x/y ghost columns (edges and corners), HLL,
+    PCM, no diffusion."""
+    p = c91_params(Nx=12, Ny=20, Nz=10, boundary_x=F.BC_REFLECTING, boundary_y=F.BC_ABSORBING,
+                   riemann_solver=F.HLL, reconstruction=F.PCM, thermal_conductivity_active=0,
+                   viscosity_active=0, time_stepping=F.TS_RK2)
+    _compare_runs(p, F.problems.c91(p), max_steps=8)
+
+
+@pytest.mark.parametrize("walls", ["fixed_temperature", "fixed_gradient"])
+def test_b02_conductivity_bitwise(gpu_available, walls):
+    """conductivity_mode=B02: the tanh profile of computeKappa
+    (ThermalConduction.h:7-22) at the face positions, and ComputeDt's
+    integer-index evaluation of it (ComputeDt.h:39-41)."""
+    bc = F.BCTC_FIXED_TEMPERATURE if walls == "fixed_temperature" else F.BCTC_FIXED_GRADIENT
+    p = Params.defaults().replace(Nx=16, Ny=8, Nz=24, boundary_x=F.BC_PERIODIC, boundary_y=F.BC_PERIODIC,
+                                  boundary_z=F.BC_REFLECTING, reconstruction=F.PLM, time_stepping=F.TS_RK2,
+                                  thermal_conductivity_active=1, thermal_conductivity_mode=F.TCM_B02,
+                                  kappa=0.05, b02_zmid=0.5, b02_kappa1=1.0, b02_kappa2=0.1,
+                                  b02_thickness=0.1, bctc_zmin=bc, bctc_zmin_value=1.2,
+                                  bctc_zmax=bc, bctc_zmax_value=0.8, tend=10.0)
+    _compare_runs(p, smooth_random(p, 6, amp_u=0.1), max_steps=8)
