@@ -106,9 +106,11 @@ int adjoint_run_t(fv3d_ctx* c, int nsteps, int K, const double* dts_dev,
       buf.alloc(&w1, ns) != hipSuccess || buf.alloc(&lam2, ns) != hipSuccess ||
       buf.alloc(&Qb, 6 * (size_t)N) != hipSuccess || buf.alloc(&tmp, ns) != hipSuccess)
     return ctx_fail(c, FV3D_ENOMEM, "adjoint: checkpoint allocation failed");
-  HIPCHK(c, hipMalloc(&grad, 2 * sizeof(double)));
+  // the two gradients, then room for one partial pair per diffusion tile
+  const size_t ngrad = 2 + 2 * (size_t)N;
+  HIPCHK(c, hipMalloc(&grad, ngrad * sizeof(double)));
   struct GradFree { double* p; ~GradFree() { hipFree(p); } } gf{grad};
-  HIPCHK(c, hipMemsetAsync(grad, 0, 2 * sizeof(double), c->stream));
+  HIPCHK(c, hipMemsetAsync(grad, 0, ngrad * sizeof(double), c->stream));
   HIPCHK(c, hipMemsetAsync(Qb, 0, 6 * (size_t)N * sizeof(real), c->stream));
 
   // adjoint of the final state, and the dt sequence on the host
@@ -158,6 +160,8 @@ int adjoint_run_t(fv3d_ctx* c, int nsteps, int K, const double* dts_dev,
   A.Qb = Qb;
   A.grad = grad;
   A.simple = std::getenv("FV3D_ADJ_SIMPLE") != nullptr;
+  A.colored = std::getenv("FV3D_ADJ_COLORED") != nullptr;
+  A.colored_diffusion = A.colored;
   for (int cp = ncp - 1; cp >= 0; --cp) {
     const int s0 = cp * K, s1 = std::min(nsteps, s0 + K);
     // recompute the segment: seg0[i] = U^{s0+i}, seg1[i] = U1 of step s0+i

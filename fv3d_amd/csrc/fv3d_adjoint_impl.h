@@ -28,6 +28,8 @@
 // so sums may associate differently from run to run.
 #pragma once
 
+#include <type_traits>
+
 #include "fv3d_device.h"
 #include "fv3d_kernels.h"
 
@@ -359,127 +361,152 @@ k_adj_faces(const real* __restrict__ U, const real* __restrict__ v, real sigma, 
 // out[0..3] = increments of rho u, rho v, rho w, E (before * mu * dt).
 // In reverse mode (REV) it hands d(out . g)/dS to SB(dk, dj, di, c, value)
 // instead, one term at a time.
+// One side s (1: lower, 2: upper) of one direction BLK (0: x, 1: y, 2: z) of
+// the viscous update of a cell (Viscosity.h:28-153, mu = 1): adds
+// sgn(s) * (flux of that side) to out[0..3] (rho u, rho v, rho w, E), or in
+// reverse mode (REV) hands d(g . sgn(s) flux)/dS to SB(dk, dj, di, c,
+// value), one term at a time.  S(dk, dj, di, c) is the cell's 3x3x3
+// velocity stencil (c = u, v, w).  The x side s=2 of cell i and the x side
+// s=1 of cell i+1 are the same flux (likewise in y); the z sides carry the
+// reference's index and component mix-ups and are not shared.
+template <typename real, bool REV, int BLK, class SF, class SBF>
+__device__ __forceinline__ void visc_side(SF&& S, const int s, real oodx, real oody, real oodz,
+                                          const real g[4], real out[4], SBF&& SB) {
+  const real f43 = (real)(4.0 / 3.0), f23 = (real)(2.0 / 3.0), h = (real)0.5, q4 = (real)0.25;
+  const real sgn = s == 1 ? (real)-1 : (real)1;
+  if (BLK == 0) {
+  // ---- X: S(1,1,s) and S(1,1,s-1), y and z differences around them
+  {
+    const real qiu = h * (S(1, 1, s, 0) + S(1, 1, s - 1, 0));
+    const real qiv = h * (S(1, 1, s, 1) + S(1, 1, s - 1, 1));
+    const real qiw = h * (S(1, 1, s, 2) + S(1, 1, s - 1, 2));
+    const real dudx = oodx * (S(1, 1, s, 0) - S(1, 1, s - 1, 0));
+    const real dvdx = oodx * (S(1, 1, s, 1) - S(1, 1, s - 1, 1));
+    const real dwdx = oodx * (S(1, 1, s, 2) - S(1, 1, s - 1, 2));
+    const real dudy = q4 * oody * (S(1, 2, s, 0) - S(1, 0, s, 0) + S(1, 2, s - 1, 0) - S(1, 0, s - 1, 0));
+    const real dvdy = q4 * oody * (S(1, 2, s, 1) - S(1, 0, s, 1) + S(1, 2, s - 1, 1) - S(1, 0, s - 1, 1));
+    const real dudz = q4 * oodz * (S(2, 1, s, 0) - S(0, 1, s, 0) + S(2, 1, s - 1, 0) - S(0, 1, s - 1, 0));
+    const real dwdz = q4 * oodz * (S(2, 1, s, 2) - S(0, 1, s, 2) + S(2, 1, s - 1, 2) - S(0, 1, s - 1, 2));
+    const real txx = f43 * dudx - f23 * (dvdy + dwdz);
+    const real txy = dvdx + dudy;
+    const real txz = dwdx + dudz;
+    if (!REV) {
+      out[0] += sgn * txx; out[1] += sgn * txy; out[2] += sgn * txz;
+      out[3] += sgn * (txx * qiu + txy * qiv + txz * qiw);
+    } else {
+      const real txxb = sgn * (g[0] + g[3] * qiu), txyb = sgn * (g[1] + g[3] * qiv),
+                 txzb = sgn * (g[2] + g[3] * qiw);
+      const real qiub = sgn * g[3] * txx * h, qivb = sgn * g[3] * txy * h, qiwb = sgn * g[3] * txz * h;
+      SB(1, 1, s, 0, qiub); SB(1, 1, s - 1, 0, qiub);
+      SB(1, 1, s, 1, qivb); SB(1, 1, s - 1, 1, qivb);
+      SB(1, 1, s, 2, qiwb); SB(1, 1, s - 1, 2, qiwb);
+      const real dudxb = f43 * txxb * oodx, dvdyb = -f23 * txxb * q4 * oody, dwdzb = -f23 * txxb * q4 * oodz;
+      const real dvdxb = txyb * oodx, dudyb = txyb * q4 * oody;
+      const real dwdxb = txzb * oodx, dudzb = txzb * q4 * oodz;
+      SB(1, 1, s, 0, dudxb); SB(1, 1, s - 1, 0, -(dudxb));
+      SB(1, 1, s, 1, dvdxb); SB(1, 1, s - 1, 1, -(dvdxb));
+      SB(1, 1, s, 2, dwdxb); SB(1, 1, s - 1, 2, -(dwdxb));
+      SB(1, 2, s, 0, dudyb); SB(1, 0, s, 0, -(dudyb)); SB(1, 2, s - 1, 0, dudyb); SB(1, 0, s - 1, 0, -(dudyb));
+      SB(1, 2, s, 1, dvdyb); SB(1, 0, s, 1, -(dvdyb)); SB(1, 2, s - 1, 1, dvdyb); SB(1, 0, s - 1, 1, -(dvdyb));
+      SB(2, 1, s, 0, dudzb); SB(0, 1, s, 0, -(dudzb)); SB(2, 1, s - 1, 0, dudzb); SB(0, 1, s - 1, 0, -(dudzb));
+      SB(2, 1, s, 2, dwdzb); SB(0, 1, s, 2, -(dwdzb)); SB(2, 1, s - 1, 2, dwdzb); SB(0, 1, s - 1, 2, -(dwdzb));
+    }
+  }
+  } else if (BLK == 1) {
+  // ---- Y: S(1,s,1) and S(1,s-1,1)
+  {
+    const real qiu = h * (S(1, s, 1, 0) + S(1, s - 1, 1, 0));
+    const real qiv = h * (S(1, s, 1, 1) + S(1, s - 1, 1, 1));
+    const real qiw = h * (S(1, s, 1, 2) + S(1, s - 1, 1, 2));
+    const real dudy = oody * (S(1, s, 1, 0) - S(1, s - 1, 1, 0));
+    const real dvdy = oody * (S(1, s, 1, 1) - S(1, s - 1, 1, 1));
+    const real dwdy = oody * (S(1, s, 1, 2) - S(1, s - 1, 1, 2));
+    const real dudx = q4 * oodx * (S(1, s, 2, 0) - S(1, s, 0, 0) + S(1, s - 1, 2, 0) - S(1, s - 1, 0, 0));
+    const real dvdx = q4 * oodx * (S(1, s, 2, 1) - S(1, s, 0, 1) + S(1, s - 1, 2, 1) - S(1, s - 1, 0, 1));
+    const real dwdz = q4 * oodz * (S(2, s, 1, 2) - S(0, s, 1, 2) + S(2, s - 1, 1, 2) - S(0, s - 1, 1, 2));
+    const real dvdz = q4 * oodz * (S(2, s, 1, 1) - S(0, s, 1, 1) + S(2, s - 1, 1, 1) - S(0, s - 1, 1, 1));
+    const real tyy = f43 * dvdy - f23 * (dudx + dwdz);
+    const real txy = dvdx + dudy;
+    const real tyz = dvdz + dwdy;
+    if (!REV) {
+      out[0] += sgn * txy; out[1] += sgn * tyy; out[2] += sgn * tyz;
+      out[3] += sgn * (txy * qiu + tyy * qiv + tyz * qiw);
+    } else {
+      const real txyb = sgn * (g[0] + g[3] * qiu), tyyb = sgn * (g[1] + g[3] * qiv),
+                 tyzb = sgn * (g[2] + g[3] * qiw);
+      const real qiub = sgn * g[3] * txy * h, qivb = sgn * g[3] * tyy * h, qiwb = sgn * g[3] * tyz * h;
+      SB(1, s, 1, 0, qiub); SB(1, s - 1, 1, 0, qiub);
+      SB(1, s, 1, 1, qivb); SB(1, s - 1, 1, 1, qivb);
+      SB(1, s, 1, 2, qiwb); SB(1, s - 1, 1, 2, qiwb);
+      const real dvdyb = f43 * tyyb * oody, dudxb = -f23 * tyyb * q4 * oodx, dwdzb = -f23 * tyyb * q4 * oodz;
+      const real dvdxb = txyb * q4 * oodx, dudyb = txyb * oody;
+      const real dvdzb = tyzb * q4 * oodz, dwdyb = tyzb * oody;
+      SB(1, s, 1, 0, dudyb); SB(1, s - 1, 1, 0, -(dudyb));
+      SB(1, s, 1, 1, dvdyb); SB(1, s - 1, 1, 1, -(dvdyb));
+      SB(1, s, 1, 2, dwdyb); SB(1, s - 1, 1, 2, -(dwdyb));
+      SB(1, s, 2, 0, dudxb); SB(1, s, 0, 0, -(dudxb)); SB(1, s - 1, 2, 0, dudxb); SB(1, s - 1, 0, 0, -(dudxb));
+      SB(1, s, 2, 1, dvdxb); SB(1, s, 0, 1, -(dvdxb)); SB(1, s - 1, 2, 1, dvdxb); SB(1, s - 1, 0, 1, -(dvdxb));
+      SB(2, s, 1, 2, dwdzb); SB(0, s, 1, 2, -(dwdzb)); SB(2, s - 1, 1, 2, dwdzb); SB(0, s - 1, 1, 2, -(dwdzb));
+      SB(2, s, 1, 1, dvdzb); SB(0, s, 1, 1, -(dvdzb)); SB(2, s - 1, 1, 1, dvdzb); SB(0, s - 1, 1, 1, -(dvdzb));
+    }
+  }
+  } else {
+  // ---- Z, with the reference's index and component mix-ups
+  // (Viscosity.h:122-133): S(s,1,1) and S(s-1,1,1)
+  {
+    const real qiu = h * (S(s, 1, 1, 0) + S(s - 1, 1, 1, 0));
+    const real qiv = h * (S(s, 1, 1, 1) + S(s - 1, 1, 1, 1));
+    const real qiw = h * (S(s, 1, 1, 2) + S(s - 1, 1, 1, 2));
+    const real dudz = oodz * (S(s, 1, 1, 0) - S(s - 1, 1, 1, 0));
+    const real dvdz = oodz * (S(s, 1, 1, 1) - S(s - 1, 1, 1, 1));
+    const real dwdz = oodz * (S(s, 1, 1, 2) - S(s - 1, 1, 1, 2));
+    const real dudx = q4 * oodx * (S(s, 1, 2, 0) - S(1, s, 0, 0) + S(s - 1, 1, 2, 0) - S(1, s - 1, 0, 0));
+    const real dwdx = q4 * oodx * (S(s, 1, 2, 1) - S(1, s, 0, 2) + S(s - 1, 1, 2, 1) - S(1, s - 1, 0, 2));
+    const real dvdy = q4 * oodz * (S(s, 2, 1, 2) - S(s, 0, 1, 1) + S(s - 1, 2, 1, 2) - S(s - 1, 0, 1, 1));
+    const real dwdy = q4 * oodz * (S(s, 2, 1, 1) - S(s, 0, 1, 2) + S(s - 1, 2, 1, 1) - S(s - 1, 0, 1, 2));
+    const real tzz = f43 * dwdz - f23 * (dudx + dvdy);
+    const real txz = dudz + dwdx;
+    const real tyz = dvdz + dwdy;
+    if (!REV) {
+      out[0] += sgn * txz; out[1] += sgn * tyz; out[2] += sgn * tzz;
+      out[3] += sgn * (txz * qiu + tyz * qiv + tzz * qiw);
+    } else {
+      const real txzb = sgn * (g[0] + g[3] * qiu), tyzb = sgn * (g[1] + g[3] * qiv),
+                 tzzb = sgn * (g[2] + g[3] * qiw);
+      const real qiub = sgn * g[3] * txz * h, qivb = sgn * g[3] * tyz * h, qiwb = sgn * g[3] * tzz * h;
+      SB(s, 1, 1, 0, qiub); SB(s - 1, 1, 1, 0, qiub);
+      SB(s, 1, 1, 1, qivb); SB(s - 1, 1, 1, 1, qivb);
+      SB(s, 1, 1, 2, qiwb); SB(s - 1, 1, 1, 2, qiwb);
+      const real dwdzb = f43 * tzzb * oodz, dudxb = -f23 * tzzb * q4 * oodx, dvdyb = -f23 * tzzb * q4 * oodz;
+      const real dudzb = txzb * oodz, dwdxb = txzb * q4 * oodx;
+      const real dvdzb = tyzb * oodz, dwdyb = tyzb * q4 * oodz;
+      SB(s, 1, 1, 0, dudzb); SB(s - 1, 1, 1, 0, -(dudzb));
+      SB(s, 1, 1, 1, dvdzb); SB(s - 1, 1, 1, 1, -(dvdzb));
+      SB(s, 1, 1, 2, dwdzb); SB(s - 1, 1, 1, 2, -(dwdzb));
+      SB(s, 1, 2, 0, dudxb); SB(1, s, 0, 0, -(dudxb)); SB(s - 1, 1, 2, 0, dudxb); SB(1, s - 1, 0, 0, -(dudxb));
+      SB(s, 1, 2, 1, dwdxb); SB(1, s, 0, 2, -(dwdxb)); SB(s - 1, 1, 2, 1, dwdxb); SB(1, s - 1, 0, 2, -(dwdxb));
+      SB(s, 2, 1, 2, dvdyb); SB(s, 0, 1, 1, -(dvdyb)); SB(s - 1, 2, 1, 2, dvdyb); SB(s - 1, 0, 1, 1, -(dvdyb));
+      SB(s, 2, 1, 1, dwdyb); SB(s, 0, 1, 2, -(dwdyb)); SB(s - 1, 2, 1, 1, dwdyb); SB(s - 1, 0, 1, 2, -(dwdyb));
+    }
+  }
+  }
+}
+
+// The viscous update of one cell, Viscosity.h:28-153 with mu = 1, as a
+// function of its 3x3x3 velocity stencil S(dk, dj, di, c) (c = u, v, w):
+// out[0..3] = increments of rho u, rho v, rho w, E (before * mu * dt).
+// In reverse mode (REV) it hands d(out . g)/dS to SB(dk, dj, di, c, value)
+// instead, one term at a time.
 template <typename real, bool REV, class SF, class SBF>
 __device__ __forceinline__ void visc_cell(SF&& S, real oodx, real oody, real oodz, const real g[4],
                                           real out[4], SBF&& SB) {
-  const real f43 = (real)(4.0 / 3.0), f23 = (real)(2.0 / 3.0), h = (real)0.5, q4 = (real)0.25;
 #pragma unroll
   for (int n = 0; n < 4; ++n) out[n] = 0;
 #pragma unroll
   for (int s = 1; s < 3; ++s) {
-    const real sgn = s == 1 ? (real)-1 : (real)1;
-    // ---- X: S(1,1,s) and S(1,1,s-1), y and z differences around them
-    {
-      const real qiu = h * (S(1, 1, s, 0) + S(1, 1, s - 1, 0));
-      const real qiv = h * (S(1, 1, s, 1) + S(1, 1, s - 1, 1));
-      const real qiw = h * (S(1, 1, s, 2) + S(1, 1, s - 1, 2));
-      const real dudx = oodx * (S(1, 1, s, 0) - S(1, 1, s - 1, 0));
-      const real dvdx = oodx * (S(1, 1, s, 1) - S(1, 1, s - 1, 1));
-      const real dwdx = oodx * (S(1, 1, s, 2) - S(1, 1, s - 1, 2));
-      const real dudy = q4 * oody * (S(1, 2, s, 0) - S(1, 0, s, 0) + S(1, 2, s - 1, 0) - S(1, 0, s - 1, 0));
-      const real dvdy = q4 * oody * (S(1, 2, s, 1) - S(1, 0, s, 1) + S(1, 2, s - 1, 1) - S(1, 0, s - 1, 1));
-      const real dudz = q4 * oodz * (S(2, 1, s, 0) - S(0, 1, s, 0) + S(2, 1, s - 1, 0) - S(0, 1, s - 1, 0));
-      const real dwdz = q4 * oodz * (S(2, 1, s, 2) - S(0, 1, s, 2) + S(2, 1, s - 1, 2) - S(0, 1, s - 1, 2));
-      const real txx = f43 * dudx - f23 * (dvdy + dwdz);
-      const real txy = dvdx + dudy;
-      const real txz = dwdx + dudz;
-      if (!REV) {
-        out[0] += sgn * txx; out[1] += sgn * txy; out[2] += sgn * txz;
-        out[3] += sgn * (txx * qiu + txy * qiv + txz * qiw);
-      } else {
-        const real txxb = sgn * (g[0] + g[3] * qiu), txyb = sgn * (g[1] + g[3] * qiv),
-                   txzb = sgn * (g[2] + g[3] * qiw);
-        const real qiub = sgn * g[3] * txx * h, qivb = sgn * g[3] * txy * h, qiwb = sgn * g[3] * txz * h;
-        SB(1, 1, s, 0, qiub); SB(1, 1, s - 1, 0, qiub);
-        SB(1, 1, s, 1, qivb); SB(1, 1, s - 1, 1, qivb);
-        SB(1, 1, s, 2, qiwb); SB(1, 1, s - 1, 2, qiwb);
-        const real dudxb = f43 * txxb * oodx, dvdyb = -f23 * txxb * q4 * oody, dwdzb = -f23 * txxb * q4 * oodz;
-        const real dvdxb = txyb * oodx, dudyb = txyb * q4 * oody;
-        const real dwdxb = txzb * oodx, dudzb = txzb * q4 * oodz;
-        SB(1, 1, s, 0, dudxb); SB(1, 1, s - 1, 0, -(dudxb));
-        SB(1, 1, s, 1, dvdxb); SB(1, 1, s - 1, 1, -(dvdxb));
-        SB(1, 1, s, 2, dwdxb); SB(1, 1, s - 1, 2, -(dwdxb));
-        SB(1, 2, s, 0, dudyb); SB(1, 0, s, 0, -(dudyb)); SB(1, 2, s - 1, 0, dudyb); SB(1, 0, s - 1, 0, -(dudyb));
-        SB(1, 2, s, 1, dvdyb); SB(1, 0, s, 1, -(dvdyb)); SB(1, 2, s - 1, 1, dvdyb); SB(1, 0, s - 1, 1, -(dvdyb));
-        SB(2, 1, s, 0, dudzb); SB(0, 1, s, 0, -(dudzb)); SB(2, 1, s - 1, 0, dudzb); SB(0, 1, s - 1, 0, -(dudzb));
-        SB(2, 1, s, 2, dwdzb); SB(0, 1, s, 2, -(dwdzb)); SB(2, 1, s - 1, 2, dwdzb); SB(0, 1, s - 1, 2, -(dwdzb));
-      }
-    }
-    // ---- Y: S(1,s,1) and S(1,s-1,1)
-    {
-      const real qiu = h * (S(1, s, 1, 0) + S(1, s - 1, 1, 0));
-      const real qiv = h * (S(1, s, 1, 1) + S(1, s - 1, 1, 1));
-      const real qiw = h * (S(1, s, 1, 2) + S(1, s - 1, 1, 2));
-      const real dudy = oody * (S(1, s, 1, 0) - S(1, s - 1, 1, 0));
-      const real dvdy = oody * (S(1, s, 1, 1) - S(1, s - 1, 1, 1));
-      const real dwdy = oody * (S(1, s, 1, 2) - S(1, s - 1, 1, 2));
-      const real dudx = q4 * oodx * (S(1, s, 2, 0) - S(1, s, 0, 0) + S(1, s - 1, 2, 0) - S(1, s - 1, 0, 0));
-      const real dvdx = q4 * oodx * (S(1, s, 2, 1) - S(1, s, 0, 1) + S(1, s - 1, 2, 1) - S(1, s - 1, 0, 1));
-      const real dwdz = q4 * oodz * (S(2, s, 1, 2) - S(0, s, 1, 2) + S(2, s - 1, 1, 2) - S(0, s - 1, 1, 2));
-      const real dvdz = q4 * oodz * (S(2, s, 1, 1) - S(0, s, 1, 1) + S(2, s - 1, 1, 1) - S(0, s - 1, 1, 1));
-      const real tyy = f43 * dvdy - f23 * (dudx + dwdz);
-      const real txy = dvdx + dudy;
-      const real tyz = dvdz + dwdy;
-      if (!REV) {
-        out[0] += sgn * txy; out[1] += sgn * tyy; out[2] += sgn * tyz;
-        out[3] += sgn * (txy * qiu + tyy * qiv + tyz * qiw);
-      } else {
-        const real txyb = sgn * (g[0] + g[3] * qiu), tyyb = sgn * (g[1] + g[3] * qiv),
-                   tyzb = sgn * (g[2] + g[3] * qiw);
-        const real qiub = sgn * g[3] * txy * h, qivb = sgn * g[3] * tyy * h, qiwb = sgn * g[3] * tyz * h;
-        SB(1, s, 1, 0, qiub); SB(1, s - 1, 1, 0, qiub);
-        SB(1, s, 1, 1, qivb); SB(1, s - 1, 1, 1, qivb);
-        SB(1, s, 1, 2, qiwb); SB(1, s - 1, 1, 2, qiwb);
-        const real dvdyb = f43 * tyyb * oody, dudxb = -f23 * tyyb * q4 * oodx, dwdzb = -f23 * tyyb * q4 * oodz;
-        const real dvdxb = txyb * q4 * oodx, dudyb = txyb * oody;
-        const real dvdzb = tyzb * q4 * oodz, dwdyb = tyzb * oody;
-        SB(1, s, 1, 0, dudyb); SB(1, s - 1, 1, 0, -(dudyb));
-        SB(1, s, 1, 1, dvdyb); SB(1, s - 1, 1, 1, -(dvdyb));
-        SB(1, s, 1, 2, dwdyb); SB(1, s - 1, 1, 2, -(dwdyb));
-        SB(1, s, 2, 0, dudxb); SB(1, s, 0, 0, -(dudxb)); SB(1, s - 1, 2, 0, dudxb); SB(1, s - 1, 0, 0, -(dudxb));
-        SB(1, s, 2, 1, dvdxb); SB(1, s, 0, 1, -(dvdxb)); SB(1, s - 1, 2, 1, dvdxb); SB(1, s - 1, 0, 1, -(dvdxb));
-        SB(2, s, 1, 2, dwdzb); SB(0, s, 1, 2, -(dwdzb)); SB(2, s - 1, 1, 2, dwdzb); SB(0, s - 1, 1, 2, -(dwdzb));
-        SB(2, s, 1, 1, dvdzb); SB(0, s, 1, 1, -(dvdzb)); SB(2, s - 1, 1, 1, dvdzb); SB(0, s - 1, 1, 1, -(dvdzb));
-      }
-    }
-    // ---- Z, with the reference's index and component mix-ups
-    // (Viscosity.h:122-133): S(s,1,1) and S(s-1,1,1)
-    {
-      const real qiu = h * (S(s, 1, 1, 0) + S(s - 1, 1, 1, 0));
-      const real qiv = h * (S(s, 1, 1, 1) + S(s - 1, 1, 1, 1));
-      const real qiw = h * (S(s, 1, 1, 2) + S(s - 1, 1, 1, 2));
-      const real dudz = oodz * (S(s, 1, 1, 0) - S(s - 1, 1, 1, 0));
-      const real dvdz = oodz * (S(s, 1, 1, 1) - S(s - 1, 1, 1, 1));
-      const real dwdz = oodz * (S(s, 1, 1, 2) - S(s - 1, 1, 1, 2));
-      const real dudx = q4 * oodx * (S(s, 1, 2, 0) - S(1, s, 0, 0) + S(s - 1, 1, 2, 0) - S(1, s - 1, 0, 0));
-      const real dwdx = q4 * oodx * (S(s, 1, 2, 1) - S(1, s, 0, 2) + S(s - 1, 1, 2, 1) - S(1, s - 1, 0, 2));
-      const real dvdy = q4 * oodz * (S(s, 2, 1, 2) - S(s, 0, 1, 1) + S(s - 1, 2, 1, 2) - S(s - 1, 0, 1, 1));
-      const real dwdy = q4 * oodz * (S(s, 2, 1, 1) - S(s, 0, 1, 2) + S(s - 1, 2, 1, 1) - S(s - 1, 0, 1, 2));
-      const real tzz = f43 * dwdz - f23 * (dudx + dvdy);
-      const real txz = dudz + dwdx;
-      const real tyz = dvdz + dwdy;
-      if (!REV) {
-        out[0] += sgn * txz; out[1] += sgn * tyz; out[2] += sgn * tzz;
-        out[3] += sgn * (txz * qiu + tyz * qiv + tzz * qiw);
-      } else {
-        const real txzb = sgn * (g[0] + g[3] * qiu), tyzb = sgn * (g[1] + g[3] * qiv),
-                   tzzb = sgn * (g[2] + g[3] * qiw);
-        const real qiub = sgn * g[3] * txz * h, qivb = sgn * g[3] * tyz * h, qiwb = sgn * g[3] * tzz * h;
-        SB(s, 1, 1, 0, qiub); SB(s - 1, 1, 1, 0, qiub);
-        SB(s, 1, 1, 1, qivb); SB(s - 1, 1, 1, 1, qivb);
-        SB(s, 1, 1, 2, qiwb); SB(s - 1, 1, 1, 2, qiwb);
-        const real dwdzb = f43 * tzzb * oodz, dudxb = -f23 * tzzb * q4 * oodx, dvdyb = -f23 * tzzb * q4 * oodz;
-        const real dudzb = txzb * oodz, dwdxb = txzb * q4 * oodx;
-        const real dvdzb = tyzb * oodz, dwdyb = tyzb * q4 * oodz;
-        SB(s, 1, 1, 0, dudzb); SB(s - 1, 1, 1, 0, -(dudzb));
-        SB(s, 1, 1, 1, dvdzb); SB(s - 1, 1, 1, 1, -(dvdzb));
-        SB(s, 1, 1, 2, dwdzb); SB(s - 1, 1, 1, 2, -(dwdzb));
-        SB(s, 1, 2, 0, dudxb); SB(1, s, 0, 0, -(dudxb)); SB(s - 1, 1, 2, 0, dudxb); SB(1, s - 1, 0, 0, -(dudxb));
-        SB(s, 1, 2, 1, dwdxb); SB(1, s, 0, 2, -(dwdxb)); SB(s - 1, 1, 2, 1, dwdxb); SB(1, s - 1, 0, 2, -(dwdxb));
-        SB(s, 2, 1, 2, dvdyb); SB(s, 0, 1, 1, -(dvdyb)); SB(s - 1, 2, 1, 2, dvdyb); SB(s - 1, 0, 1, 1, -(dvdyb));
-        SB(s, 2, 1, 1, dwdyb); SB(s, 0, 1, 2, -(dwdyb)); SB(s - 1, 2, 1, 1, dwdyb); SB(s - 1, 0, 1, 2, -(dwdyb));
-      }
-    }
+    visc_side<real, REV, 0>(S, s, oodx, oody, oodz, g, out, SB);
+    visc_side<real, REV, 1>(S, s, oodx, oody, oodz, g, out, SB);
+    visc_side<real, REV, 2>(S, s, oodx, oody, oodz, g, out, SB);
   }
 }
 
@@ -620,6 +647,24 @@ k_adj_diffusion(const real* __restrict__ U, const real* __restrict__ v, real sig
 // contributions of its faces or cells into an LDS accumulator of the same
 // window, and adds the window to Qb with one global atomic per value.
 // ---------------------------------------------------------------------------
+// Tiles are swept in colour passes: along each axis alternate tiles take
+// colours 0 and 1, and the last of an odd count takes colour 2.  A window
+// reaches at most half a tile beyond its tile, so the windows of one pass
+// are disjoint and the two ends of a periodic axis never run together: each
+// pass adds its window sums to Qb with plain read-modify-writes, and only
+// ghost cells (whose source may be another window's cell) use atomics, after
+// a barrier.
+inline int tile_ncolors(int n) { return n == 1 ? 1 : ((n & 1) ? 3 : 2); }
+// tiles of colour c along an axis of n tiles, and the b-th of them
+inline int tile_count(int c, int n) {
+  if (n == 1) return 1;
+  if (c == 2) return 1;
+  return (n & 1) ? (n - 1) / 2 : n / 2;
+}
+__device__ __forceinline__ int tile_of(int b, int c, int n) {
+  return c < 0 ? b : (c == 2 ? n - 1 : 2 * b + c);  // c < 0: every tile, atomics only
+}
+
 constexpr int AF_W = 64;  // faces (x) or lanes (y, z) per tile row
 constexpr int AF_R = 4;   // rows (x: y rows; y, z: face rows) per tile
 
@@ -634,7 +679,7 @@ struct FaceTile {
 template <typename real, int DIR, int RECON, int RS>
 __global__ void __launch_bounds__(256)
 k_adj_faces_tiled(const real* __restrict__ U, const real* __restrict__ v, real sigma, real* Qb,
-                  double dt, DevParams P) {
+                  double dt, DevParams P, int color, int ntiles) {
   using T = FaceTile<real, DIR>;
   __shared__ real sq[5][T::NWIN];
   __shared__ real sa[5][T::NWIN];
@@ -643,10 +688,11 @@ k_adj_faces_tiled(const real* __restrict__ U, const real* __restrict__ v, real s
   const int bc = DIR == FV3D_IX ? P.bcx : (DIR == FV3D_IY ? P.bcy : P.bcz);
   const long long stride = DIR == FV3D_IX ? 1 : (DIR == FV3D_IY ? P.sy : P.sz);
   // tile origin: x0 (lanes or faces), d0 (first face along DIR), fixed coords
+  // the grid covers the tiles of one colour along DIR (all tiles across it)
   int i0, j0, k0;
-  if (DIR == FV3D_IX) { i0 = blockIdx.x * AF_W; j0 = blockIdx.y * AF_R; k0 = blockIdx.z; }
-  else if (DIR == FV3D_IY) { i0 = blockIdx.x * AF_W; j0 = blockIdx.y * AF_R; k0 = blockIdx.z; }
-  else { i0 = blockIdx.x * AF_W; j0 = blockIdx.y; k0 = blockIdx.z * AF_R; }
+  if (DIR == FV3D_IX) { i0 = tile_of(blockIdx.x, color, ntiles) * AF_W; j0 = blockIdx.y * AF_R; k0 = blockIdx.z; }
+  else if (DIR == FV3D_IY) { i0 = blockIdx.x * AF_W; j0 = tile_of(blockIdx.y, color, ntiles) * AF_R; k0 = blockIdx.z; }
+  else { i0 = blockIdx.x * AF_W; j0 = blockIdx.y; k0 = tile_of(blockIdx.z, color, ntiles) * AF_R; }
   const int d0 = DIR == FV3D_IX ? i0 : (DIR == FV3D_IY ? j0 : k0);
   // window cell w -> (coordinate along DIR, base offset of the other two)
   auto win = [&](int w, int& g, bool& inside, long long& base) {
@@ -678,10 +724,11 @@ k_adj_faces_tiled(const real* __restrict__ U, const real* __restrict__ v, real s
   const real coef = (real)((double)sigma * dt / d);
   // face fm (relative to d0, -1..AF_*-1) in tile row/lane t: stencil window
   // cells along DIR fm+1 .. fm+4
-  auto face = [&](int t, int fm) {
+  // computes face fm of row/lane t; fills its contributions qb to the
+  // window cells wc (the stencil a, b, c, d); false if there is no such face
+  auto face = [&](int t, int fm, real (&qb)[4][5], int (&wc)[4]) -> bool {
     const int m = d0 + fm;
-    if (m < -1 || m > Nd - 1) return;
-    int wc[4];
+    if (m < -1 || m > Nd - 1) return false;
 #pragma unroll
     for (int o = 0; o < 4; ++o)
       wc[o] = DIR == FV3D_IX ? t * T::NWC + (fm + 1 + o) : (fm + 1 + o) * T::NWC + t;
@@ -689,7 +736,7 @@ k_adj_faces_tiled(const real* __restrict__ U, const real* __restrict__ v, real s
     {
       int g; bool inside; long long base;
       win(wc[1], g, inside, base);
-      if (!inside) return;
+      if (!inside) return false;
       real q[4][5];
 #pragma unroll
       for (int o = 0; o < 4; ++o)
@@ -727,7 +774,6 @@ k_adj_faces_tiled(const real* __restrict__ U, const real* __restrict__ v, real s
       else hllc_vjp(qL, qR, (real)P.gamma, (real)P.entho, Fb, Lb, Rb);
       swap_dir<DIR>(Lb);
       swap_dir<DIR>(Rb);
-      real qb[4][5];
 #pragma unroll
       for (int n = 0; n < 5; ++n) { qb[0][n] = 0; qb[1][n] = Lb[n]; qb[2][n] = Rb[n]; qb[3][n] = 0; }
       if (RECON == FV3D_PLM) {
@@ -737,35 +783,58 @@ k_adj_faces_tiled(const real* __restrict__ U, const real* __restrict__ v, real s
         slope_vjp(selb, ssb, qb[0], qb[1], qb[2]);
         slope_vjp(selc, ssc, qb[1], qb[2], qb[3]);
       }
-#pragma unroll
-      for (int o = 0; o < 4; ++o)
-#pragma unroll
-        for (int n = 0; n < 5; ++n)
-          if (qb[o][n] != (real)0) unsafeAtomicAdd(&sa[n][wc[o]], qb[o][n]);
     }
+    return true;
   };
-  // AF_W x AF_R faces, one per thread; the tile at the low domain edge also
-  // owns face -1/2 of every row
-  if (DIR == FV3D_IX) {
-    face(tid / AF_W, tid % AF_W);
-    if (d0 == 0 && tid < AF_R) face(tid, -1);
-  } else {
-    face(tid % AF_W, tid / AF_W);
-    if (d0 == 0 && tid < AF_W) face(tid, -1);
-  }
-  __syncthreads();
-
-  for (int w = tid; w < T::NWIN; w += blockDim.x) {
-    int g; bool inside; long long base;
-    win(w, g, inside, base);
-    if (!inside || g < -2 || g > Nd + 1) continue;
-    bool fl;
-    const long long src = base + (long long)map_coord(g, Nd, bc, fl) * stride;
+  // AF_W x AF_R faces, one per thread.  Their contributions are summed in
+  // four passes, one per stencil role: within a pass every face writes a
+  // different window cell, so plain LDS updates suffice (no atomics).  The
+  // tile at the low domain edge also owns face -1/2 of every row, summed in
+  // four more passes.
+  {
+    real qb[4][5];
+    int wc[4];
+    const bool ok = DIR == FV3D_IX ? face(tid / AF_W, tid % AF_W, qb, wc)
+                                   : face(tid % AF_W, tid / AF_W, qb, wc);
 #pragma unroll
-    for (int n = 0; n < 5; ++n) {
-      real x = sa[n][w];
-      if (fl && n == 1 + DIR) x = -x;
-      atomic_add(&Qb[src + n * P.sf], x);
+    for (int r = 0; r < 4; ++r) {
+      if (ok)
+#pragma unroll
+        for (int n = 0; n < 5; ++n) sa[n][wc[r]] += qb[r][n];
+      __syncthreads();
+    }
+  }
+  if (d0 == 0) {
+    real qb[4][5];
+    int wc[4];
+    const int nx = DIR == FV3D_IX ? AF_R : AF_W;
+    const bool ok = tid < nx && face(tid, -1, qb, wc);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if (ok)
+#pragma unroll
+        for (int n = 0; n < 5; ++n) sa[n][wc[r]] += qb[r][n];
+      __syncthreads();
+    }
+  }
+
+  for (int ghosts = 0; ghosts < 2; ++ghosts) {
+    if (ghosts) __syncthreads();  // this block's own updates land first
+    for (int w = tid; w < T::NWIN; w += blockDim.x) {
+      int g; bool inside; long long base;
+      win(w, g, inside, base);
+      if (!inside || g < -2 || g > Nd + 1) continue;
+      const bool ghost = g < 0 || g >= Nd;
+      if (color < 0 ? ghosts == 1 : ghost != (ghosts == 1)) continue;
+      bool fl;
+      const long long src = base + (long long)map_coord(g, Nd, bc, fl) * stride;
+#pragma unroll
+      for (int n = 0; n < 5; ++n) {
+        real x = sa[n][w];
+        if (fl && n == 1 + DIR) x = -x;
+        if (ghost || color < 0) atomic_add(&Qb[src + n * P.sf], x);
+        else if (x != (real)0) Qb[src + n * P.sf] += x;
+      }
     }
   }
 }
@@ -775,12 +844,15 @@ constexpr int AD_X = 32, AD_Y = 4, AD_Z = 4;  // diffusion tile
 template <typename real, bool TC, bool VISC>
 __global__ void __launch_bounds__(AD_X * AD_Y * AD_Z)
 k_adj_diffusion_tiled(const real* __restrict__ U, const real* __restrict__ v, real sigma,
-                      real* Qb, double dt, DevParams P, double* grad) {
+                      real* Qb, double dt, DevParams P, double* grad, int3 color, int3 ntiles) {
   constexpr int WX = AD_X + 2, WY = AD_Y + 2, WZ = AD_Z + 2, NW = WX * WY * WZ;
   __shared__ real sq[4][NW];  // u, v, w, T of the window (ghost-mapped)
   __shared__ real sa[4][NW];  // their adjoints
   const int tid = threadIdx.x;
-  const int i0 = blockIdx.x * AD_X, j0 = blockIdx.y * AD_Y, k0 = blockIdx.z * AD_Z;
+  // the grid covers the tiles of one colour
+  const int i0 = tile_of(blockIdx.x, color.x, ntiles.x) * AD_X;
+  const int j0 = tile_of(blockIdx.y, color.y, ntiles.y) * AD_Y;
+  const int k0 = tile_of(blockIdx.z, color.z, ntiles.z) * AD_Z;
   const real gm1 = (real)P.gm1;
   auto wcoord = [&](int w, int& i, int& j, int& k) {
     i = i0 - 1 + w % WX;
@@ -807,6 +879,16 @@ k_adj_diffusion_tiled(const real* __restrict__ U, const real* __restrict__ v, re
   const int ti = tid % AD_X, tj = (tid / AD_X) % AD_Y, tk = tid / (AD_X * AD_Y);
   const int i = i0 + ti, j = j0 + tj, k = k0 + tk;
   double gk = 0.0, gm = 0.0;
+  // this cell's contributions to its 3x3x3 neighbourhood, summed below in
+  // one pass per offset (plain LDS updates: within a pass every thread
+  // writes a different window cell)
+  real Sb[3][3][3][3], Tb[3][3][3];
+#pragma unroll
+  for (int e = 0; e < 27; ++e) {
+    (&Tb[0][0][0])[e] = (real)0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) (&Sb[0][0][0][0])[3 * e + c] = (real)0;
+  }
   if (i < P.Nx && j < P.Ny && k < P.Nz) {
     const long long cell = i + (long long)j * P.sy + (long long)k * P.sz;
     auto W = [&](int dk, int dj, int di) {  // window index of offset (di-1, dj-1, dk-1)
@@ -843,58 +925,127 @@ k_adj_diffusion_tiled(const real* __restrict__ U, const real* __restrict__ v, re
       for (int o = 0; o < 7; ++o) dE1 += c[o] * sq[3][nb[o]];
       gk = (double)Eb * (double)dE1;
       const real kf = (real)P.kface;
-#pragma unroll
-      for (int o = 0; o < 7; ++o) {
-        const real x = Eb * kf * c[o];
-        if (x != (real)0) unsafeAtomicAdd(&sa[3][nb[o]], x);
-      }
+      Tb[1][1][1] += Eb * kf * c[0];
+      Tb[1][1][0] += Eb * kf * c[1];
+      Tb[1][1][2] += Eb * kf * c[2];
+      Tb[1][0][1] += Eb * kf * c[3];
+      Tb[1][2][1] += Eb * kf * c[4];
+      Tb[0][1][1] += Eb * kf * c[5];
+      Tb[2][1][1] += Eb * kf * c[6];
     }
     if (VISC) {
+      // per side: x and y faces once (side 2 of the lower cell, carrying
+      // both cells' adjoints; side 1 only at the low domain edge), z sides
+      // per cell
       const real oodx = (real)1 / (real)P.dx, oody = (real)1 / (real)P.dy, oodz = (real)1 / (real)P.dz;
-      real vb[4], out[4];
+      const real sdt = sigma * rdt, mu = (real)P.mu;
+      real gi[4];
 #pragma unroll
-      for (int c = 0; c < 4; ++c) vb[c] = sigma * v[cell + (c + 1) * P.sf];
+      for (int c = 0; c < 4; ++c) gi[c] = sdt * v[cell + (c + 1) * P.sf];
       auto Sget = [&](int dk, int dj, int di, int c) -> real { return sq[c][W(dk, dj, di)]; };
-      auto Sacc = [&](int dk, int dj, int di, int c, real x) {
-        if (x != (real)0) unsafeAtomicAdd(&sa[c][W(dk, dj, di)], x);
-      };
-      visc_cell<real, false>(Sget, oodx, oody, oodz, vb, out, Sacc);
+      auto Sacc = [&](int dk, int dj, int di, int c, real x) { Sb[dk][dj][di][c] += x; };
       double gmu = 0.0;
+      auto side = [&](auto blk, int sd, const real gw[4]) {
+        constexpr int B = decltype(blk)::value;
+        real out[4] = {0, 0, 0, 0}, gm4[4];
+        visc_side<real, false, B>(Sget, sd, oodx, oody, oodz, gw, out, Sacc);
 #pragma unroll
-      for (int c = 0; c < 4; ++c) gmu += (double)vb[c] * (double)out[c];
-      gm = gmu * dt;
-      real g[4];
-      const real mdt = (real)P.mu * rdt;
+        for (int c = 0; c < 4; ++c) { gmu += (double)gw[c] * (double)out[c]; gm4[c] = gw[c] * mu; }
+        visc_side<real, true, B>(Sget, sd, oodx, oody, oodz, gm4, out, Sacc);
+      };
+      using BX = std::integral_constant<int, 0>;
+      using BY = std::integral_constant<int, 1>;
+      using BZ = std::integral_constant<int, 2>;
+      real gf[4];
 #pragma unroll
-      for (int c = 0; c < 4; ++c) g[c] = vb[c] * mdt;
-      visc_cell<real, true>(Sget, oodx, oody, oodz, g, out, Sacc);
+      for (int c = 0; c < 4; ++c) gf[c] = gi[c] - (i + 1 < P.Nx ? sdt * v[cell + 1 + (c + 1) * P.sf] : (real)0);
+      side(BX{}, 2, gf);
+      if (i == 0) side(BX{}, 1, gi);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) gf[c] = gi[c] - (j + 1 < P.Ny ? sdt * v[cell + P.sy + (c + 1) * P.sf] : (real)0);
+      side(BY{}, 2, gf);
+      if (j == 0) side(BY{}, 1, gi);
+      side(BZ{}, 1, gi);
+      side(BZ{}, 2, gi);
+      gm = gmu;
     }
   }
+#pragma unroll
+  for (int dk = 0; dk < 3; ++dk)
+#pragma unroll
+    for (int dj = 0; dj < 3; ++dj)
+#pragma unroll
+      for (int di = 0; di < 3; ++di) {
+        const int w = (ti + di) + WX * ((tj + dj) + WY * (tk + dk));
+        if (VISC) {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) sa[c][w] += Sb[dk][dj][di][c];
+        }
+        if (TC) sa[3][w] += Tb[dk][dj][di];
+        __syncthreads();
+      }
   __syncthreads();
-  for (int w = tid; w < NW; w += blockDim.x) {
-    int wi, wj, wk;
-    wcoord(w, wi, wj, wk);
-    if (wi < -1 || wi > P.Nx || wj < -1 || wj > P.Ny || wk < -1 || wk > P.Nz) continue;
-    bool fx, fy, fz;
-    const int si = map_coord(wi, P.Nx, P.bcx, fx), sj = map_coord(wj, P.Ny, P.bcy, fy),
-              sk = map_coord(wk, P.Nz, P.bcz, fz);
-    const long long src = si + (long long)sj * P.sy + (long long)sk * P.sz;
-    if (VISC) {
-      atomic_add(&Qb[src + 1 * P.sf], fx ? -sa[0][w] : sa[0][w]);
-      atomic_add(&Qb[src + 2 * P.sf], fy ? -sa[1][w] : sa[1][w]);
-      atomic_add(&Qb[src + 3 * P.sf], fz ? -sa[2][w] : sa[2][w]);
+  for (int ghosts = 0; ghosts < 2; ++ghosts) {
+    if (ghosts) __syncthreads();  // this block's own updates land first
+    for (int w = tid; w < NW; w += blockDim.x) {
+      int wi, wj, wk;
+      wcoord(w, wi, wj, wk);
+      if (wi < -1 || wi > P.Nx || wj < -1 || wj > P.Ny || wk < -1 || wk > P.Nz) continue;
+      const bool ghost = wi < 0 || wi >= P.Nx || wj < 0 || wj >= P.Ny || wk < 0 || wk >= P.Nz;
+      if (ghost != (ghosts == 1) && color.x >= 0) continue;
+      if (color.x < 0 && ghosts == 1) continue;
+      bool fx, fy, fz;
+      const int si = map_coord(wi, P.Nx, P.bcx, fx), sj = map_coord(wj, P.Ny, P.bcy, fy),
+                sk = map_coord(wk, P.Nz, P.bcz, fz);
+      const long long src = si + (long long)sj * P.sy + (long long)sk * P.sz;
+      auto put = [&](long long a, real x) {
+        if (ghost || color.x < 0) atomic_add(&Qb[a], x);
+        else if (x != (real)0) Qb[a] += x;
+      };
+      if (VISC) {
+        put(src + 1 * P.sf, fx ? -sa[0][w] : sa[0][w]);
+        put(src + 2 * P.sf, fy ? -sa[1][w] : sa[1][w]);
+        put(src + 3 * P.sf, fz ? -sa[2][w] : sa[2][w]);
+      }
+      if (TC) put(src + 5 * P.sf, sa[3][w]);
     }
-    if (TC) atomic_add(&Qb[src + 5 * P.sf], sa[3][w]);
   }
+  // parameter gradients: one partial per tile (summed by k_adj_reduce; a
+  // global atomic per wave on one address would serialise every wave)
+  __shared__ double sred[2][AD_X * AD_Y * AD_Z / 64];
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
     gk += __shfl_xor(gk, off);
     gm += __shfl_xor(gm, off);
   }
-  if ((threadIdx.x & 63) == 0) {
-    if (TC && gk != 0.0) unsafeAtomicAdd(&grad[0], gk);
-    if (VISC && gm != 0.0) unsafeAtomicAdd(&grad[1], gm);
+  if ((tid & 63) == 0) { sred[0][tid >> 6] = gk; sred[1][tid >> 6] = gm; }
+  __syncthreads();
+  if (tid == 0) {
+    double a = 0.0, b = 0.0;
+    for (int w = 0; w < AD_X * AD_Y * AD_Z / 64; ++w) { a += sred[0][w]; b += sred[1][w]; }
+    const long long t = (i0 / AD_X) + (long long)ntiles.x * ((j0 / AD_Y) + (long long)ntiles.y * (k0 / AD_Z));
+    grad[2 + 2 * t] = a;
+    grad[3 + 2 * t] = b;
   }
+}
+
+// grad[0..1] += the per-tile partials grad[2 + 2t], grad[3 + 2t]
+template <typename real>  // one instance per precision's translation unit
+__global__ void __launch_bounds__(256) k_adj_reduce(double* grad, long long ntiles) {
+  __shared__ double s[2][256];
+  double a = 0.0, b = 0.0;
+  for (long long t = threadIdx.x; t < ntiles; t += 256) { a += grad[2 + 2 * t]; b += grad[3 + 2 * t]; }
+  s[0][threadIdx.x] = a;
+  s[1][threadIdx.x] = b;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      s[0][threadIdx.x] += s[0][threadIdx.x + h];
+      s[1][threadIdx.x] += s[1][threadIdx.x + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { grad[0] += s[0][0]; grad[1] += s[1][0]; }
 }
 
 // out = sigma*v + J_c2p(U)^T Qb + alpha*extra; Qb cleared
@@ -949,9 +1100,21 @@ static hipError_t adj_faces_dir(const AdjLaunch<real>& L, hipStream_t s) {
   if (DIR == FV3D_IX) grid = dim3((P.Nx + AF_W - 1) / AF_W, (P.Ny + AF_R - 1) / AF_R, P.Nz);
   else if (DIR == FV3D_IY) grid = dim3((P.Nx + AF_W - 1) / AF_W, (P.Ny + AF_R - 1) / AF_R, P.Nz);
   else grid = dim3((P.Nx + AF_W - 1) / AF_W, P.Ny, (P.Nz + AF_R - 1) / AF_R);
-  hipLaunchKernelGGL((k_adj_faces_tiled<real, DIR, RECON, RS>), grid, dim3(AF_W * AF_R), 0, s, L.U,
-                     L.v, L.sigma, L.Qb, L.dt, P);
-  return hipGetLastError();
+  const int nt = DIR == FV3D_IX ? grid.x : (DIR == FV3D_IY ? grid.y : grid.z);
+  if (!L.colored) {
+    hipLaunchKernelGGL((k_adj_faces_tiled<real, DIR, RECON, RS>), grid, dim3(AF_W * AF_R), 0, s, L.U,
+                       L.v, L.sigma, L.Qb, L.dt, P, -1, nt);
+    return hipGetLastError();
+  }
+  for (int c = 0; c < tile_ncolors(nt); ++c) {
+    dim3 g = grid;
+    (DIR == FV3D_IX ? g.x : (DIR == FV3D_IY ? g.y : g.z)) = tile_count(c, nt);
+    hipLaunchKernelGGL((k_adj_faces_tiled<real, DIR, RECON, RS>), g, dim3(AF_W * AF_R), 0, s, L.U,
+                       L.v, L.sigma, L.Qb, L.dt, P, c, nt);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 template <typename real, int RECON, int RS>
@@ -972,8 +1135,22 @@ static void adj_diffusion(const AdjLaunch<real>& L, hipStream_t s) {
     return;
   }
   const dim3 grid((P.Nx + AD_X - 1) / AD_X, (P.Ny + AD_Y - 1) / AD_Y, (P.Nz + AD_Z - 1) / AD_Z);
-  hipLaunchKernelGGL((k_adj_diffusion_tiled<real, TC, VISC>), grid, dim3(AD_X * AD_Y * AD_Z), 0, s,
-                     L.U, L.v, L.sigma, L.Qb, L.dt, P, L.grad);
+  const int3 nt = make_int3(grid.x, grid.y, grid.z);
+  const long long ntl = (long long)nt.x * nt.y * nt.z;
+  if (!L.colored || !L.colored_diffusion) {
+    hipLaunchKernelGGL((k_adj_diffusion_tiled<real, TC, VISC>), grid, dim3(AD_X * AD_Y * AD_Z), 0, s,
+                       L.U, L.v, L.sigma, L.Qb, L.dt, P, L.grad, make_int3(-1, -1, -1), nt);
+    hipLaunchKernelGGL((k_adj_reduce<real>), dim3(1), dim3(256), 0, s, L.grad, ntl);
+    return;
+  }
+  for (int cz = 0; cz < tile_ncolors(nt.z); ++cz)
+    for (int cy = 0; cy < tile_ncolors(nt.y); ++cy)
+      for (int cx = 0; cx < tile_ncolors(nt.x); ++cx) {
+        const dim3 g(tile_count(cx, nt.x), tile_count(cy, nt.y), tile_count(cz, nt.z));
+        hipLaunchKernelGGL((k_adj_diffusion_tiled<real, TC, VISC>), g, dim3(AD_X * AD_Y * AD_Z), 0, s,
+                           L.U, L.v, L.sigma, L.Qb, L.dt, P, L.grad, make_int3(cx, cy, cz), nt);
+      }
+  hipLaunchKernelGGL((k_adj_reduce<real>), dim3(1), dim3(256), 0, s, L.grad, ntl);
 }
 
 // one adjoint stage: Qb must be zero on entry and is zero on return

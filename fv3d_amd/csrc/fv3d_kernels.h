@@ -79,11 +79,16 @@ struct AdjLaunch {
   real* Qb;
   real* out;
   double dt;
-  double* grad;
+  double* grad;  // [2 + 2 * tiles]: dJ/dkappa, dJ/dmu, then per-tile partials (scratch)
   DevParams P;
   int recon, riemann;
   bool tc, visc;
-  bool simple = false;  // one thread per face / cell with global atomics (check path)
+  bool simple = false;   // one thread per face / cell with global atomics (check path)
+  // tiled kernels: flush windows in colour passes with plain updates instead
+  // of one launch with global atomics (measured slower on the 128^3 box:
+  // each pass holds too few tiles to fill the GPU)
+  bool colored = false;
+  bool colored_diffusion = false;
 };
 template <typename real>
 hipError_t launch_adjoint_stage(const AdjLaunch<real>& L, hipStream_t s);

@@ -144,10 +144,14 @@ def test_tiled_adjoint_equals_per_face_form(gpu_available, name, monkeypatch):
     (FV3D_ADJ_SIMPLE): the same sums, associated differently."""
     pr = Problem(base_params(N=20, **CONFIGS[name]), nsteps=5)
     _, gt, kt, mt = pr.adjoint(K=2)
+    monkeypatch.setenv("FV3D_ADJ_COLORED", "1")  # colour passes instead of atomic flushes
+    _, gc, kc, mc = pr.adjoint(K=2)
+    monkeypatch.delenv("FV3D_ADJ_COLORED")
     monkeypatch.setenv("FV3D_ADJ_SIMPLE", "1")
     _, gs, ks, ms = pr.adjoint(K=2)
-    assert np.abs(gt - gs).max() <= 1e-12 * np.abs(gs).max()
-    assert kt == pytest.approx(ks, rel=1e-12, abs=1e-300) and mt == pytest.approx(ms, rel=1e-12, abs=1e-300)
+    for g, k, m in [(gt, kt, mt), (gc, kc, mc)]:
+        assert np.abs(g - gs).max() <= 1e-12 * np.abs(gs).max()
+        assert k == pytest.approx(ks, rel=1e-12, abs=1e-300) and m == pytest.approx(ms, rel=1e-12, abs=1e-300)
 
 
 @pytest.mark.gpu
