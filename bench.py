@@ -207,8 +207,9 @@ def adjoint_form(F, n=128, steps=200, K=20, ndirs=16, precision=None, fd=True):
     Reports the adjoint's device time (checkpointed forward pass + backward
     pass) over the frozen forward run's, and central differences
     (eps = 1e-6) of J along `ndirs` unit-norm random directions of the
-    initial density field against the adjoint's dJ/drho0, plus dJ/dkappa and
-    dJ/dmu against their own central differences."""
+    initial density field (Gaussian random fields with the density's own
+    k^-2 spectrum) against the adjoint's dJ/drho0, plus dJ/dkappa and dJ/dmu
+    against their own central differences."""
     from fv3d_amd import problems
     precision = F.FP64 if precision is None else precision
     p = F.Params.defaults().replace(
@@ -264,7 +265,8 @@ def adjoint_form(F, n=128, steps=200, K=20, ndirs=16, precision=None, fd=True):
         eps = 1e-6
         rel = []
         for i in range(ndirs):
-            d = rng.standard_normal(g.shape)
+            # perturbations of the same class as the density field (P(k) ~ k^-2), unit norm
+            d = problems.spectral_field(rng, g.shape, -2.0)
             d /= np.linalg.norm(d)
             Jp = J_of(d0[..., 0] + eps * d)[0]
             Jm = J_of(d0[..., 0] - eps * d)[0]

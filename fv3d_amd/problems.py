@@ -214,7 +214,7 @@ def rt_ensemble(base, nbox: int, seed: int = 12345, sides=(32, 48, 64, 96)):
     return out
 
 
-def _spectral_field(rng, shape, slope):
+def spectral_field(rng, shape, slope):
     """Real Gaussian random field with power spectrum P(k) ~ k^slope on a
     periodic grid (k in units of the box), zero mean, unit rms."""
     white = rng.standard_normal(shape)
@@ -258,9 +258,9 @@ def turbulent_box(p, seed: int = 12345, rho_amp: float = 0.05, v_rms: float = 0.
     seeded with `seed`."""
     rng = np.random.default_rng(seed)
     shape = (p.Nz, p.Ny, p.Nx)
-    g = _spectral_field(rng, shape, slope)
+    g = spectral_field(rng, shape, slope)
     u, v, w = _solenoidal_velocity(rng, shape, slope)
-    gp = _spectral_field(rng, shape, slope) if p_amp else 0.0
+    gp = spectral_field(rng, shape, slope) if p_amp else 0.0
     Q = _empty(p)
     d = _dom(Q, p)
     d[..., 0] = 1.0 + rho_amp * g
