@@ -123,7 +123,8 @@ __device__ __forceinline__ void face_flux4(const Prim<real>& a, const Prim<real>
 // marched by the whole workgroup.  Called by the single-box kernel (one tile
 // per workgroup) and by the persistent multi-box kernel (a work list of
 // (box, tile) items).
-template <typename real, int TY, int RECON, int RS, bool TC, bool VISC, int MODE>
+template <typename real, int TY, int RECON, int RS, bool TC, bool VISC, int MODE,
+          bool C91Z = false>
 __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const real* Urk,
                                            real* Uout, const real* __restrict__ Kin, real* Cio,
                                            const unsigned long long* __restrict__ inv_cur,
@@ -181,8 +182,10 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
   const real dt = (real)dtd;
   if (inv_zero && tile == 0 && threadIdx.x == 0) *inv_zero = 0ull;
 
-  // C91 z ghosts come precomputed (fillC91, BoundaryConditions.h:86-134):
-  // the column's value with the x/y reflections applied
+  // C91 z ghosts (C91Z: boundary_z == BC_C91; a template switch, since a
+  // run-time test in these loads costs the plain kernels ~8%) come
+  // precomputed (fillC91, BoundaryConditions.h:86-134): the column's value
+  // with the x/y reflections applied
   auto c91_ghost = [&](int kk, long long xy, bool gx, bool gy) -> Prim<real> {
     const real* g = (const real*)P.c91g + (long long)(kk < 0 ? kk + 2 : kk - P.Nz + 2) * 5 * P.sz + xy;
     Prim<real> q{g[0], g[P.sz], g[2 * P.sz], g[3 * P.sz], g[4 * P.sz]};
@@ -191,13 +194,13 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
     return q;
   };
   auto load_plane = [&](int kk) -> Prim<real> {
-    if (P.bcz == FV3D_BC_C91 && (kk < 0 || kk >= P.Nz)) return c91_ghost(kk, sxy, fx, fy);
+    if (C91Z && (kk < 0 || kk >= P.Nz)) return c91_ghost(kk, sxy, fx, fy);
     bool fz;
     const int sk = map_coord(kk, P.Nz, P.bcz, fz);
     return load_prim<real, MODE>(Uin, Kin, sxy + (long long)sk * P.sz, P.sf, gm1, fx, fy, fz);
   };
   auto load_halo = [&](int kk) -> Prim<real> {
-    if (P.bcz == FV3D_BC_C91 && (kk < 0 || kk >= P.Nz)) return c91_ghost(kk, hxy, hfx, hfy);
+    if (C91Z && (kk < 0 || kk >= P.Nz)) return c91_ghost(kk, hxy, hfx, hfy);
     bool fz;
     const int sk = map_coord(kk, P.Nz, P.bcz, fz);
     return load_prim<real, MODE>(Uin, Kin, hxy + (long long)sk * P.sz, P.sf, gm1, hfx, hfy, fz);
@@ -586,7 +589,7 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
   }
 }
 
-template <typename real, int TY, int RECON, int RS, bool TC, bool VISC, int MODE>
+template <typename real, int TY, int RECON, int RS, bool TC, bool VISC, int MODE, bool C91Z>
 __global__ void __launch_bounds__(64 * (TY + 4))
 stage_kernel(const real* __restrict__ Uin, const real* Urk, real* Uout,
              const real* __restrict__ Kin, real* Cio,
@@ -598,30 +601,30 @@ stage_kernel(const real* __restrict__ Uin, const real* Urk, real* Uout,
   const int nb = gridDim.x, b = blockIdx.x;
   const int q8 = nb >> 3, r8 = nb & 7, xcd = b & 7, loc = b >> 3;
   const int tile = xcd * q8 + (xcd < r8 ? xcd : r8) + loc;
-  stage_tile<real, TY, RECON, RS, TC, VISC, MODE>(Uin, Urk, Uout, Kin, Cio, inv_cur, inv_next,
+  stage_tile<real, TY, RECON, RS, TC, VISC, MODE, C91Z>(Uin, Urk, Uout, Kin, Cio, inv_cur, inv_next,
                                                   inv_zero, health, P, tile);
 }
 
 // ---------------------------------------------------------------------------
 // launcher
 // ---------------------------------------------------------------------------
-template <typename real, int TY, int RECON, int RS, bool TC, bool VISC, int MODE>
+template <typename real, int TY, int RECON, int RS, bool TC, bool VISC, int MODE, bool C91Z>
 static hipError_t launch_t(const StageLaunch<real>& L, hipStream_t s) {
   DevParams P = L.P;
   const int blocks = P.tiles_x * P.tiles_y * P.tiles_z;
-  hipLaunchKernelGGL((stage_kernel<real, TY, RECON, RS, TC, VISC, MODE>), dim3(blocks),
+  hipLaunchKernelGGL((stage_kernel<real, TY, RECON, RS, TC, VISC, MODE, C91Z>), dim3(blocks),
                      dim3(64 * (TY + 4)), 0, s, L.Uin, L.Urk, L.Uout, L.Kin, L.Cio, L.inv_cur,
                      L.inv_next, L.inv_zero, L.health, P);
   return hipGetLastError();
 }
 
-template <typename real, int RECON, int RS, int MODE>
+template <typename real, int RECON, int RS, int MODE, bool C91Z>
 static hipError_t launch_phys(const StageLaunch<real>& L, hipStream_t s) {
   const bool tc = L.tc, vi = L.visc;
-  if (tc && vi) return launch_t<real, STAGE_TY_VISC, RECON, RS, true, true, MODE>(L, s);
-  if (tc) return launch_t<real, STAGE_TY, RECON, RS, true, false, MODE>(L, s);
-  if (vi) return launch_t<real, STAGE_TY_VISC, RECON, RS, false, true, MODE>(L, s);
-  return launch_t<real, STAGE_TY, RECON, RS, false, false, MODE>(L, s);
+  if (tc && vi) return launch_t<real, STAGE_TY_VISC, RECON, RS, true, true, MODE, C91Z>(L, s);
+  if (tc) return launch_t<real, STAGE_TY, RECON, RS, true, false, MODE, C91Z>(L, s);
+  if (vi) return launch_t<real, STAGE_TY_VISC, RECON, RS, false, true, MODE, C91Z>(L, s);
+  return launch_t<real, STAGE_TY, RECON, RS, false, false, MODE, C91Z>(L, s);
 }
 
 // fillC91 (BoundaryConditions.h:86-134) for the z-ghost planes of every
@@ -684,10 +687,16 @@ static hipError_t launch_mode(const StageLaunch<real>& L, hipStream_t s) {
   }
   const bool plm = L.recon == FV3D_PLM;
   const bool hll = L.riemann == FV3D_HLL;
-  if (plm && hll) return launch_phys<real, FV3D_PLM, FV3D_HLL, MODE>(L, s);
-  if (plm) return launch_phys<real, FV3D_PLM, FV3D_HLLC, MODE>(L, s);
-  if (hll) return launch_phys<real, FV3D_PCM, FV3D_HLL, MODE>(L, s);
-  return launch_phys<real, FV3D_PCM, FV3D_HLLC, MODE>(L, s);
+  if (L.P.bcz == FV3D_BC_C91) {
+    if (plm && hll) return launch_phys<real, FV3D_PLM, FV3D_HLL, MODE, true>(L, s);
+    if (plm) return launch_phys<real, FV3D_PLM, FV3D_HLLC, MODE, true>(L, s);
+    if (hll) return launch_phys<real, FV3D_PCM, FV3D_HLL, MODE, true>(L, s);
+    return launch_phys<real, FV3D_PCM, FV3D_HLLC, MODE, true>(L, s);
+  }
+  if (plm && hll) return launch_phys<real, FV3D_PLM, FV3D_HLL, MODE, false>(L, s);
+  if (plm) return launch_phys<real, FV3D_PLM, FV3D_HLLC, MODE, false>(L, s);
+  if (hll) return launch_phys<real, FV3D_PCM, FV3D_HLL, MODE, false>(L, s);
+  return launch_phys<real, FV3D_PCM, FV3D_HLLC, MODE, false>(L, s);
 }
 
 }  // namespace fv3d_hip
