@@ -198,6 +198,52 @@ def ensemble_form(F, nbox, steps, tune_cache=None):
             "l1_rho_fp32_vs_fp64_max": max(l1), "l1_rho_fp32_vs_fp64_mean": float(np.mean(l1))}
 
 
+def stratified_form(F, steps=2000):
+    """Stratified convection with conduction and viscosity: 256x256x128 on
+    [0,2]x[0,2]x[0,1] (dx = 1/128), polytrope m=1 under g=1 (T = 1 + z/2,
+    rho = T, p = T^2), gamma 5/3, kappa 0.07, mu 0.005, velocity noise 1e-3,
+    HLL + PLM, SSP-RK2, periodic x/y, reflecting z with fixed temperatures
+    1.0 / 1.5 at the walls.  CFL 0.4: at 0.8 the reference's conduction dt
+    is unstable and the FP64 run itself blows up.  Runs FP64 and FP32
+    (compensated RK2) for `steps` steps, reading the dt triple of every step,
+    and reports their throughput and the L1 error of FP32 against FP64 per
+    field."""
+    p = F.Params.defaults().replace(
+        Nx=256, Ny=256, Nz=128, xmax=2.0, ymax=2.0, boundary_x=F.BC_PERIODIC, boundary_y=F.BC_PERIODIC,
+        boundary_z=F.BC_REFLECTING, reconstruction=F.PLM, riemann_solver=F.HLL, time_stepping=F.TS_RK2,
+        gamma0=5.0 / 3.0, gravity=1, g=1.0, CFL=0.4, thermal_conductivity_active=1, kappa=0.07,
+        bctc_zmin=F.BCTC_FIXED_TEMPERATURE, bctc_zmin_value=1.0, bctc_zmax=F.BCTC_FIXED_TEMPERATURE,
+        bctc_zmax_value=1.5, viscosity_active=1, mu=0.005)
+    from fv3d_amd import problems
+    Q0 = problems.polytrope_convection(p)
+    cells = p.Nx * p.Ny * p.Nz
+    out = {"grid": [p.Nx, p.Ny, p.Nz], "steps": steps, "CFL": p.CFL}
+    res = {}
+    for name, prec in (("fp64", F.FP64), ("fp32_kahan", F.FP32)):
+        with F.Simulation(p, prec) as sim:
+            sim.set_prim(Q0)
+            trip = np.zeros((steps, 3))
+            t0 = time.perf_counter()
+            for s_ in range(steps):
+                dt, inv = sim.compute_dt()
+                trip[s_] = [1.0 / inv[0], 1.0 / inv[1], 1.0 / inv[2]]
+                sim.step()
+            sim.synchronize()
+            wall = time.perf_counter() - t0
+            res[name] = (sim.get_prim(with_ghosts=False), trip)
+            tot, _ = sim.time_steps(20)  # device-only rate on the final state
+        out[name] = {"Gcells_incl_dt_readback": round(cells * steps / wall / 1e9, 4),
+                     "Gcells_device": round(cells * 20 / (tot * 1e-3) / 1e9, 4),
+                     "dt_hyp_first_last": [trip[0, 0], trip[-1, 0]], "dt_TC": trip[0, 1], "dt_visc": trip[0, 2]}
+        print(f"stratified: {name} done ({wall:.1f} s)", file=sys.stderr, flush=True)
+    q64, t64 = res["fp64"]
+    q32, t32 = res["fp32_kahan"]
+    out["l1_fp32_vs_fp64"] = {f: float(np.mean(np.abs(q32[..., i] - q64[..., i])))
+                              for i, f in enumerate(["rho", "u", "v", "w", "prs"])}
+    out["dt_hyp_max_rel_diff_fp32_vs_fp64"] = float(np.max(np.abs(t32[:, 0] / t64[:, 0] - 1.0)))
+    return out
+
+
 def adjoint_form(F, n=128, steps=200, K=20, ndirs=16, precision=None, fd=True):
     """The discrete adjoint workload: n^3 periodic box, gamma 5/3, PLM +
     HLLC, SSP-RK2, CFL 0.4, conduction kappa=0.02 and viscosity mu=0.01;
@@ -297,6 +343,7 @@ def main():
     ap.add_argument("--forms", type=int, default=1, help="also time the FP32 forms")
     ap.add_argument("--ensemble", type=int, default=512, help="boxes of the batched ensemble (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=500)
+    ap.add_argument("--stratified", type=int, default=2000, help="steps of the stratified-convection form (0: skip)")
     ap.add_argument("--adjoint", type=int, default=128, help="grid of the adjoint workload (0: skip)")
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--adjoint-dirs", type=int, default=16)
@@ -367,6 +414,8 @@ def main():
         }
     if args.ensemble > 0:
         out.setdefault("forms", {})["fp32_ensemble_rt"] = ensemble_form(F, args.ensemble, args.ensemble_steps)
+    if args.stratified > 0:
+        out.setdefault("forms", {})["stratified_tc_visc"] = stratified_form(F, args.stratified)
     if args.adjoint > 0:
         out.setdefault("forms", {})["adjoint_fp64"] = adjoint_form(F, args.adjoint, args.adjoint_steps,
                                                                    ndirs=args.adjoint_dirs)
