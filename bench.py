@@ -298,13 +298,26 @@ def adjoint_form(F, n=128, steps=200, K=20, ndirs=16, precision=None, fd=True):
     lamT[..., 0] = 2.0 * (rT - target)
     lam0, dk, dm = sim.adjoint(dts, lamT, K)
     t_fwd_pass, t_bwd, t_adj = sim.adjoint_times()
+    # the same run with every segment kept in HBM from the forward pass
+    # (no recomputation): 10 segments of 41 states at 128^3 FP64 = 34 GB
+    sim.set_prim(Q0)
+    real_bytes = 8 if precision == F.FP64 else 4
+    store = (2 * K + 1) * 5 * n ** 3 * real_bytes * ((steps + K - 1) // K)
+    lam0s, dks, dms = sim.adjoint(dts, lamT, K, store_bytes=store)
+    ts_fwd, ts_bwd, ts_adj = sim.adjoint_times()
+    stored_rel = float(np.abs(lam0s - lam0).max() / np.abs(lam0).max())
     g = lam0[..., 0] + (lam0[..., 1:4] * vel).sum(-1) + lam0[..., 4] * 0.5 * (vel * vel).sum(-1)
     print(f"adjoint: forward {min(fwd_ms):.1f} ms, adjoint {t_adj:.1f} ms", file=sys.stderr, flush=True)
     out = {"grid": [n, n, n], "steps": steps, "checkpoint_every": K,
            "dtype": "f64" if precision == F.FP64 else "f32",
            "forward_ms": round(min(fwd_ms), 3), "adjoint_ms": round(t_adj, 3),
            "adjoint_forward_pass_ms": round(t_fwd_pass, 3), "adjoint_backward_ms": round(t_bwd, 3),
-           "adjoint_over_forward": round(t_adj / min(fwd_ms), 3), "J": J0,
+           "adjoint_over_forward": round(t_adj / min(fwd_ms), 3),
+           "stored_segments": {"store_gb": round(store / 1e9, 2), "adjoint_ms": round(ts_adj, 3),
+                               "forward_pass_ms": round(ts_fwd, 3), "backward_ms": round(ts_bwd, 3),
+                               "adjoint_over_forward": round(ts_adj / min(fwd_ms), 3),
+                               "max_rel_diff_vs_recomputed": stored_rel},
+           "J": J0,
            "dJ_dkappa": dk, "dJ_dmu": dm}
     if fd and ndirs > 0:
         rng = np.random.default_rng(7)

@@ -169,6 +169,7 @@ def lib() -> ctypes.CDLL:
         "fv3d_forward_frozen": (ctypes.c_int, [vp, ctypes.c_int, dp, P(ctypes.c_float)]),
         "fv3d_adjoint_run": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, dp, dp, dp, dp, dp]),
         "fv3d_adjoint_times": (ctypes.c_int, [vp, P(ctypes.c_float)]),
+        "fv3d_set_adjoint_store": (ctypes.c_int, [vp, ctypes.c_size_t]),
         "fv3d_batch_create": (ctypes.c_int, [ctypes.c_int, P(Params), ctypes.c_int, ctypes.c_int, P(vp)]),
         "fv3d_batch_destroy": (ctypes.c_int, [vp]),
         "fv3d_batch_last_error": (ctypes.c_char_p, [vp]),
@@ -311,10 +312,13 @@ class Simulation:
         _check(lib().fv3d_forward_frozen(self._ctx, len(dts), _dptr(dts), ctypes.byref(ms)), self._ctx)
         return ms.value
 
-    def adjoint(self, dts, dJ_dUT: np.ndarray, K: int = 20):
+    def adjoint(self, dts, dJ_dUT: np.ndarray, K: int = 20, store_bytes: int = 0):
         """dJ/dU0 (Nz, Ny, Nx, 5), dJ/dkappa, dJ/dmu for J(U_T) after the
         frozen-dt run of len(dts) steps from the current state, given
-        dJ/dU_T (Nz, Ny, Nx, 5).  Checkpoints every K steps."""
+        dJ/dU_T (Nz, Ny, Nx, 5).  Checkpoints every K steps; segments that
+        fit in store_bytes of device memory are kept from the forward pass
+        instead of being recomputed."""
+        _check(lib().fv3d_set_adjoint_store(self._ctx, int(store_bytes)), self._ctx)
         dts = np.ascontiguousarray(dts, dtype=np.float64)
         lamT = np.ascontiguousarray(dJ_dUT, dtype=np.float64)
         if lamT.shape != self.params.shape_domain:

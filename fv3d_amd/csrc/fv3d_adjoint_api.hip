@@ -10,7 +10,9 @@
 //   Euler: lam <- lam + dt J_R(U^s)^T lam
 // States are checkpointed every K steps in the forward pass; each segment
 // is recomputed from its checkpoint (keeping U^s and U1 of every step) and
-// then swept backwards.  The parameter gradients dJ/dkappa and dJ/dmu are
+// then swept backwards.  Segments that fit in the context's store budget
+// (fv3d_set_adjoint_store; the last segments first) keep those states from
+// the forward pass itself and are not recomputed.  The parameter gradients dJ/dkappa and dJ/dmu are
 // the sums over all stages of (adjoint of the stage output) . dS/dtheta.
 #include <algorithm>
 #include <cmath>
@@ -97,12 +99,24 @@ int adjoint_run_t(fv3d_ctx* c, int nsteps, int K, const double* dts_dev,
   const size_t ns = 5 * (size_t)N;
   const bool rk2 = c->p.time_stepping == FV3D_TS_RK2;
   const int ncp = (nsteps + K - 1) / K;
+  // segments kept from the forward pass: cp >= first_kept.  A kept segment
+  // holds U^s for its K+1 step boundaries, then (RK2) the K stage states.
+  const size_t seg_states = rk2 ? 2 * (size_t)K + 1 : (size_t)K + 1;
+  const size_t seg_bytes = seg_states * ns * sizeof(real);
+  const int keep = (int)std::min<size_t>((size_t)ncp, c->adj_store_bytes / seg_bytes);
+  const int first_kept = ncp - keep;
   AdjBuffers<real> buf;
   real *ckpt = nullptr, *seg0 = nullptr, *seg1 = nullptr, *lam = nullptr, *w1 = nullptr,
-       *lam2 = nullptr, *Qb = nullptr, *tmp = nullptr;
+       *lam2 = nullptr, *Qb = nullptr, *tmp = nullptr, *store = nullptr;
   double* grad = nullptr;
-  if (buf.alloc(&ckpt, ns * ncp) != hipSuccess || buf.alloc(&seg0, ns * (K + 1)) != hipSuccess ||
-      (rk2 && buf.alloc(&seg1, ns * K) != hipSuccess) || buf.alloc(&lam, ns) != hipSuccess ||
+  auto kept0 = [&](int cp) { return store + (size_t)(cp - first_kept) * seg_states * ns; };
+  auto kept1 = [&](int cp) { return kept0(cp) + (size_t)(K + 1) * ns; };
+  if (keep > 0 && buf.alloc(&store, seg_states * ns * keep) != hipSuccess)
+    return ctx_fail(c, FV3D_ENOMEM, "adjoint: segment store allocation failed");
+  const bool recompute = first_kept > 0;
+  if (buf.alloc(&ckpt, ns * ncp) != hipSuccess ||
+      (recompute && buf.alloc(&seg0, ns * (K + 1)) != hipSuccess) ||
+      (recompute && rk2 && buf.alloc(&seg1, ns * K) != hipSuccess) || buf.alloc(&lam, ns) != hipSuccess ||
       buf.alloc(&w1, ns) != hipSuccess || buf.alloc(&lam2, ns) != hipSuccess ||
       buf.alloc(&Qb, 6 * (size_t)N) != hipSuccess || buf.alloc(&tmp, ns) != hipSuccess)
     return ctx_fail(c, FV3D_ENOMEM, "adjoint: checkpoint allocation failed");
@@ -130,13 +144,27 @@ int adjoint_run_t(fv3d_ctx* c, int nsteps, int K, const double* dts_dev,
   HIPCHK(c, hipEventCreate(&e2));
   HIPCHK(c, hipEventRecord(e0, c->stream));
   // ---- forward pass: checkpoints every K steps (the steps after the last
-  // checkpoint are recomputed by the backward pass anyway)
+  // checkpoint are recomputed by the backward pass anyway, unless kept)
   HIPCHK(c, hipMemcpyAsync(ckpt, c->A, ns * sizeof(real), hipMemcpyDeviceToDevice, c->stream));
   {
     real* cur = tmp;
     HIPCHK(c, hipMemcpyAsync(cur, c->A, ns * sizeof(real), hipMemcpyDeviceToDevice, c->stream));
     real* other = lam2;  // scratch until the backward pass
-    for (int s = 0; s < (ncp - 1) * K; ++s) {
+    const int fwd_end = keep > 0 ? nsteps : (ncp - 1) * K;
+    for (int s = 0; s < fwd_end; ++s) {
+      const int cp = s / K, i = s - cp * K;
+      if (cp >= first_kept) {  // a kept segment: step inside the store
+        real* S0 = kept0(cp);
+        if (i == 0)
+          HIPCHK(c, hipMemcpyAsync(S0, cur, ns * sizeof(real), hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, frozen_step<real>(c, dts_dev + s, S0 + (size_t)i * ns,
+                                    rk2 ? kept1(cp) + (size_t)i * ns : nullptr,
+                                    S0 + (size_t)(i + 1) * ns));
+        if (i == K - 1 && s + 1 < nsteps)
+          HIPCHK(c, hipMemcpyAsync(cur, S0 + (size_t)K * ns, ns * sizeof(real),
+                                   hipMemcpyDeviceToDevice, c->stream));
+        continue;
+      }
       if (rk2) {
         HIPCHK(c, frozen_step<real>(c, dts_dev + s, cur, w1, cur));
       } else {
@@ -164,26 +192,33 @@ int adjoint_run_t(fv3d_ctx* c, int nsteps, int K, const double* dts_dev,
   A.colored_diffusion = A.colored;
   for (int cp = ncp - 1; cp >= 0; --cp) {
     const int s0 = cp * K, s1 = std::min(nsteps, s0 + K);
-    // recompute the segment: seg0[i] = U^{s0+i}, seg1[i] = U1 of step s0+i
-    HIPCHK(c, hipMemcpyAsync(seg0, ckpt + (size_t)cp * ns, ns * sizeof(real),
-                             hipMemcpyDeviceToDevice, c->stream));
-    for (int s = s0; s < s1 - 1; ++s) {
-      const int i = s - s0;
-      HIPCHK(c, frozen_step<real>(c, dts_dev + s, seg0 + (size_t)i * ns,
-                                  rk2 ? seg1 + (size_t)i * ns : nullptr, seg0 + (size_t)(i + 1) * ns));
-    }
-    if (rk2) {  // the last step's stage state (its output is not needed)
-      StageLaunch<real> L = frozen_stage<real>(c, dts_dev + (s1 - 1));
-      L.Uin = seg0 + (size_t)(s1 - 1 - s0) * ns;
-      L.Uout = seg1 + (size_t)(s1 - 1 - s0) * ns;
-      HIPCHK(c, launch_stage<real>(L, c->stream));
+    // the segment's states: S0[i] = U^{s0+i}, S1[i] = U1 of step s0+i
+    real* S0 = seg0;
+    real* S1 = seg1;
+    if (cp >= first_kept) {
+      S0 = kept0(cp);
+      S1 = kept1(cp);
+    } else {  // recompute the segment from its checkpoint
+      HIPCHK(c, hipMemcpyAsync(S0, ckpt + (size_t)cp * ns, ns * sizeof(real),
+                               hipMemcpyDeviceToDevice, c->stream));
+      for (int s = s0; s < s1 - 1; ++s) {
+        const int i = s - s0;
+        HIPCHK(c, frozen_step<real>(c, dts_dev + s, S0 + (size_t)i * ns,
+                                    rk2 ? S1 + (size_t)i * ns : nullptr, S0 + (size_t)(i + 1) * ns));
+      }
+      if (rk2) {  // the last step's stage state (its output is not needed)
+        StageLaunch<real> L = frozen_stage<real>(c, dts_dev + (s1 - 1));
+        L.Uin = S0 + (size_t)(s1 - 1 - s0) * ns;
+        L.Uout = S1 + (size_t)(s1 - 1 - s0) * ns;
+        HIPCHK(c, launch_stage<real>(L, c->stream));
+      }
     }
     for (int s = s1 - 1; s >= s0; --s) {
       const int i = s - s0;
       A.dt = hdts[s];
-      real* U0 = seg0 + (size_t)i * ns;
+      real* U0 = S0 + (size_t)i * ns;
       if (rk2) {
-        A.U = seg1 + (size_t)i * ns; A.v = lam; A.sigma = (real)0.5; A.extra = nullptr;
+        A.U = S1 + (size_t)i * ns; A.v = lam; A.sigma = (real)0.5; A.extra = nullptr;
         A.alpha = (real)0; A.out = w1;
         HIPCHK(c, launch_adjoint_stage<real>(A, c->stream));
         A.U = U0; A.v = w1; A.sigma = (real)1; A.extra = lam; A.alpha = (real)0.5; A.out = lam2;
@@ -282,6 +317,12 @@ int fv3d_adjoint_run(fv3d_ctx* c, int nsteps, int K, const double* dts, const do
   soa_domain_to_aos(c->p, lam0.data(), 0, dJ_dU0);
   if (dJ_dkappa) *dJ_dkappa = grads[0];
   if (dJ_dmu) *dJ_dmu = grads[1];
+  return FV3D_OK;
+}
+
+int fv3d_set_adjoint_store(fv3d_ctx* c, size_t bytes) {
+  if (!c) return FV3D_EINVAL;
+  c->adj_store_bytes = bytes;
   return FV3D_OK;
 }
 

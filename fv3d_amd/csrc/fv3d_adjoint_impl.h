@@ -76,33 +76,37 @@ __device__ __forceinline__ void hllc_vjp(const real L[5], const real R[5], real 
   const real sub = Fb[0] * sr + Fb[1] * (real)2 * sr * su + tv * sr + Fb[4] * (E + sp);
   const real spb = Fb[1] + Fb[4] * su;
   const real Eb = Fb[4] * su;
-  if (vw_left) { Lb[2] += Fb[2] * F0; Lb[3] += Fb[3] * F0; }
-  else { Rb[2] += Fb[2] * F0; Rb[3] += Fb[3] * F0; }
+  // Every side choice below is a select on values: a branch that adds to Lb
+  // on one side and Rb on the other becomes a select between the two
+  // arrays' addresses, which puts both in scratch memory.
+  const real z = (real)0;
+  const real t2 = Fb[2] * F0, t3 = Fb[3] * F0;
+  Lb[2] += vw_left ? t2 : z; Lb[3] += vw_left ? t3 : z;
+  Rb[2] += vw_left ? z : t2; Rb[3] += vw_left ? z : t3;
 
-  real ELb = 0, ERb = 0, SLb = 0, SRb = 0, uSb = 0, pSb = 0;
-  if (star) {
-    uSb += sub;
-    pSb += spb;
-    real Ab = srb * rK / D, Db = -srb * rS / D;
-    const real rKb = srb * A / D;
-    const real NEb = Eb / D;
-    Db -= Eb * ES / D;
-    Ab += NEb * EK;
-    const real EKb = NEb * A;
-    const real pKb = -NEb * uK;
-    real uKb = -NEb * pK;
-    pSb += NEb * uS;
-    uSb += NEb * pS;
-    const real SKb = Ab + Db;
-    uKb -= Ab;
-    uSb -= Db;
-    if (kl) { Lb[0] += rKb; Lb[1] += uKb; Lb[4] += pKb; ELb += EKb; SLb += SKb; }
-    else { Rb[0] += rKb; Rb[1] += uKb; Rb[4] += pKb; ERb += EKb; SRb += SKb; }
-  } else if (left) {
-    Lb[0] += srb; Lb[1] += sub; Lb[4] += spb; ELb += Eb;
-  } else {
-    Rb[0] += srb; Rb[1] += sub; Rb[4] += spb; ERb += Eb;
-  }
+  // star region: the adjoints of rS = rK A / D and ES = (A EK - pK uK +
+  // pS uS) / D with A = SK - uK, D = SK - uS (evaluated in every region,
+  // used only in the star region)
+  real Ab = srb * rK / D, Db = -srb * rS / D;
+  const real rKb = srb * A / D;
+  const real NEb = Eb / D;
+  Db -= Eb * ES / D;
+  Ab += NEb * EK;
+  const real EKb = NEb * A;
+  const real pKb = -NEb * uK;
+  const real uKb = -NEb * pK - Ab;
+  const real SKb = Ab + Db;
+  const real uSb = star ? sub + NEb * pS - Db : z;
+  const real pSb = star ? spb + NEb * uS : z;
+  // the side that receives the state adjoints: K in the star region, else
+  // the upwind side
+  const bool toL = star ? kl : left;
+  const real c0 = star ? rKb : srb, c1 = star ? uKb : sub, c4 = star ? pKb : spb;
+  const real cE = star ? EKb : Eb, cS = star ? SKb : z;
+  Lb[0] += toL ? c0 : z; Lb[1] += toL ? c1 : z; Lb[4] += toL ? c4 : z;
+  Rb[0] += toL ? z : c0; Rb[1] += toL ? z : c1; Rb[4] += toL ? z : c4;
+  const real ELb = toL ? cE : z, ERb = toL ? z : cE;
+  real SLb = toL ? cS : z, SRb = toL ? z : cS;
   const real NUb = uSb / den, NPb = pSb / den;
   const real denb = -(uSb * uS + pSb * pS) / den;
   real rcRb = NUb * uR + denb, rcLb = NUb * uL + denb;
@@ -124,8 +128,8 @@ __device__ __forceinline__ void hllc_vjp(const real L[5], const real R[5], real 
   SRb += rcRb * rR;
   Rb[1] -= rcRb * rR;
   const real cmaxb = SRb - SLb;
-  if (uL < uR) Lb[1] += SLb; else Rb[1] += SLb;
-  if (uL > uR) Lb[1] += SRb; else Rb[1] += SRb;
+  Lb[1] += (uL < uR ? SLb : z) + (uL > uR ? SRb : z);
+  Rb[1] += (uL < uR ? z : SLb) + (uL > uR ? z : SRb);
   const real cLb = cL >= cR ? cmaxb : (real)0, cRb = cL >= cR ? (real)0 : cmaxb;
   Lb[4] += cLb * gamma / ((real)2 * rL * cL);
   Lb[0] -= cLb * cL / ((real)2 * rL);

@@ -31,6 +31,7 @@ struct fv3d_ctx {
   bool have_stage = false;  // B holds the reference's post-update Q source
   fv3d_hip::DevParams dp;
   float adj_ms[3] = {0, 0, 0};  // last adjoint run: forward pass, backward pass, total
+  size_t adj_store_bytes = 0;   // device memory for segments kept from the forward pass
   std::string err;
 };
 

@@ -237,7 +237,7 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
     fetch_halo(k0, cz);
   }
   real Tm = 0, T0 = 0, T1 = 0;
-  if (TC) { Tm = Pm1.p / Pm1.r; T0 = P0.p / P0.r; T1 = P1.p / P1.r; }
+  if (TC) { Tm = qdiv(Pm1.p, Pm1.r); T0 = qdiv(P0.p, P0.r); T1 = qdiv(P1.p, P1.r); }
   real um = Pm1.u, vm = Pm1.v, wm = Pm1.w;
 
   const real kf = (real)P.kface;
@@ -256,7 +256,7 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
     };
     const Prim<real> P2 = load_plane(k + 2);
     real T2 = 0;
-    if (TC) T2 = P2.p / P2.r;
+    if (TC) T2 = qdiv(P2.p, P2.r);
     Prim<real> H{}, Hm{}, Hp{};
     if (hown) {
       bool hfz;
@@ -279,7 +279,7 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
     if (hown) {
       sq[0][hr][hc] = H.r; sq[1][hr][hc] = H.u; sq[2][hr][hc] = H.v; sq[3][hr][hc] = H.w;
       sq[4][hr][hc] = H.p;
-      if (TC) sT[hr][hc] = H.p / H.r;
+      if (TC) sT[hr][hc] = qdiv(H.p, H.r);
       if (VISC) {
         svm[0][hr][hc] = Hm.u; svm[1][hr][hc] = Hm.v; svm[2][hr][hc] = Hm.w;
         svp[0][hr][hc] = Hp.u; svp[1][hr][hc] = Hp.v; svp[2][hr][hc] = Hp.w;
@@ -366,12 +366,12 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
 #pragma unroll
       for (int n = 0; n < 5; ++n) {
         const real t = (sfx[n][r - 2][lane] - sfx[n][r - 2][lane + 1]) * dt;
-        u[n] = u[n] + (P.pow2x ? t * (real)P.idx : t / (real)P.dx);
+        u[n] = u[n] + over_h(t, P.pow2x, P.idx, P.dx);
       }
 #pragma unroll
       for (int n = 0; n < 5; ++n) {
         const real t = (sfy[n][r - 2][lane] - sfy[n][r - 1][lane]) * dt;
-        u[n] = u[n] + (P.pow2y ? t * (real)P.idy : t / (real)P.dy);
+        u[n] = u[n] + over_h(t, P.pow2y, P.idy, P.dy);
       }
       real zl[5], zh[5];
 #pragma unroll
@@ -391,7 +391,7 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
 #pragma unroll
       for (int n = 0; n < 5; ++n) {
         const real t = (zl[n] - zh[n]) * dt;
-        u[n] = u[n] + (P.pow2z ? t * (real)P.idz : t / (real)P.dz);
+        u[n] = u[n] + over_h(t, P.pow2z, P.idz, P.dz);
       }
       if (P.gravity) {  // Update.h:148-151
         u[3] = u[3] + dt * P0.r * (real)P.g;
@@ -404,9 +404,9 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
         const real TL = sT[r][c - 1], TR = sT[r][c + 1];
         const real TU = sT[r - 1][c], TD = sT[r + 1][c];
         const real TF = Tm, TB = T1;
-        auto ddx = [&](real v) { return P.pow2x ? v * (real)P.idx : v / (real)P.dx; };
-        auto ddy = [&](real v) { return P.pow2y ? v * (real)P.idy : v / (real)P.dy; };
-        auto ddz = [&](real v) { return P.pow2z ? v * (real)P.idz : v / (real)P.dz; };
+        auto ddx = [&](real v) { return over_h(v, P.pow2x, P.idx, P.dx); };
+        auto ddy = [&](real v) { return over_h(v, P.pow2y, P.idy, P.dy); };
+        auto ddz = [&](real v) { return over_h(v, P.pow2z, P.idz, P.dz); };
         // face conductivities: constant, or B02's z profile (kappaL..kappaB)
         real kxy = kf, kzF = kf, kzB = kf;
         if (P.kztab) {

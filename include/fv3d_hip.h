@@ -131,6 +131,14 @@ int fv3d_forward_frozen(fv3d_ctx* ctx, int nsteps, const double* dts, float* ms)
 int fv3d_adjoint_run(fv3d_ctx* ctx, int nsteps, int K, const double* dts, const double* dJ_dUT,
                      double* dJ_dU0, double* dJ_dkappa, double* dJ_dmu);
 int fv3d_adjoint_times(const fv3d_ctx* ctx, float ms[3]);
+/* Device memory (bytes) that later fv3d_adjoint_run calls on this context
+ * may use to keep whole segments from the forward pass: a kept segment's
+ * U^s and stage states are stored as the forward pass computes them, and
+ * the backward pass sweeps it without recomputing it.  The last segments
+ * are kept first; 0 (the default) recomputes every segment.  A segment of
+ * K RK2 steps takes (2K+1) * 5 * Nx*Ny*Nz * sizeof(real) bytes (K+1 for
+ * Euler).  The result does not depend on it. */
+int fv3d_set_adjoint_store(fv3d_ctx* ctx, size_t bytes);
 
 /* ---- ensembles of independent boxes (no counterpart in the reference) ----
  *

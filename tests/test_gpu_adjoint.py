@@ -80,7 +80,7 @@ class Problem:
             r = sim.get_prim(with_ghosts=False)[..., 0]
         return float(np.sum((r - self.target) ** 2))
 
-    def adjoint(self, p=None, precision=F.FP64, K=4):
+    def adjoint(self, p=None, precision=F.FP64, K=4, store_bytes=0):
         p = p or self.p
         with F.Simulation(p, precision) as sim:
             sim.set_prim(self.Q0)
@@ -88,7 +88,7 @@ class Problem:
             rT = self.rho_T(p, dom(self.Q0), precision)
             lamT = np.zeros(p.shape_domain)
             lamT[..., 0] = 2.0 * (rT - self.target)
-            g, dk, dm = sim.adjoint(self.dts, lamT, K)
+            g, dk, dm = sim.adjoint(self.dts, lamT, K, store_bytes=store_bytes)
             assert np.array_equal(sim.get_cons(with_ghosts=False), U0)  # state untouched
         return U0, g, dk, dm
 
@@ -135,6 +135,23 @@ def test_adjoint_checkpoint_interval_does_not_change_result(gpu_available):
     for g in (g9, g4):
         assert np.abs(g - g1).max() <= 1e-12 * scale  # only atomic summation order differs
     assert k4 == pytest.approx(k1, rel=1e-12) and m9 == pytest.approx(m1, rel=1e-12)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["periodic_hllc_plm_rk2_tc_visc", "pcm_hllc_euler_visc"])
+def test_adjoint_segment_store_does_not_change_result(gpu_available, name):
+    """Segments kept from the forward pass (fv3d_set_adjoint_store) against
+    recomputed ones: 9 steps in segments of 4, 4 and 1; a budget of two
+    segments keeps the last two, a large one keeps all three."""
+    p = base_params(**CONFIGS[name])
+    pr = Problem(p, nsteps=9)
+    rk2 = p.time_stepping == F.TS_RK2
+    seg = (2 * 4 + 1 if rk2 else 4 + 1) * 5 * p.Nx * p.Ny * p.Nz * 8
+    _, g0, k0, m0 = pr.adjoint(K=4)
+    for store in (seg - 1, 2 * seg, 1 << 34):
+        _, g, k, m = pr.adjoint(K=4, store_bytes=store)
+        assert np.abs(g - g0).max() <= 1e-12 * np.abs(g0).max(), store
+        assert k == pytest.approx(k0, rel=1e-12, abs=1e-300) and m == pytest.approx(m0, rel=1e-12)
 
 
 @pytest.mark.gpu
