@@ -50,15 +50,17 @@ __device__ __forceinline__ void hllc_vjp(const real L[5], const real R[5], real 
   const real cmax = fmax(cL, cR);
   const real SL = fmin(uL, uR) - cmax, SR = fmax(uL, uR) + cmax;
   const real rcL = rL * (uL - SL), rcR = rR * (SR - uR);
-  const real den = rcR + rcL;
-  const real uS = (rcR * uR + rcL * uL + (pL - pR)) / den;
-  const real pS = (rcR * pL + rcL * pR + rcL * rcR * (uL - uR)) / den;
+  // Quotients over den and D go through one reciprocal each: within an ulp
+  // of the forward values, and the branch tests (signs, SL, SR) are exact.
+  const real den = rcR + rcL, iden = (real)1 / den;
+  const real uS = (rcR * uR + rcL * uL + (pL - pR)) * iden;
+  const real pS = (rcR * pL + rcL * pR + rcL * rcR * (uL - uR)) * iden;
   const bool kl = uS > (real)0;
   const real rK = kl ? rL : rR, uK = kl ? uL : uR, pK = kl ? pL : pR, EK = kl ? EL : ER,
              SK = kl ? SL : SR;
-  const real A = SK - uK, D = SK - uS;
-  const real rS = rK * A / D;
-  const real ES = (A * EK - pK * uK + pS * uS) / D;
+  const real A = SK - uK, D = SK - uS, iD = (real)1 / D;
+  const real rS = rK * A * iD;
+  const real ES = (A * EK - pK * uK + pS * uS) * iD;
   const bool left = SL > (real)0;
   const bool star = !left && (kl || SR > (real)0);
   const real sr = star ? rS : (left ? rL : rR);
@@ -87,10 +89,10 @@ __device__ __forceinline__ void hllc_vjp(const real L[5], const real R[5], real 
   // star region: the adjoints of rS = rK A / D and ES = (A EK - pK uK +
   // pS uS) / D with A = SK - uK, D = SK - uS (evaluated in every region,
   // used only in the star region)
-  real Ab = srb * rK / D, Db = -srb * rS / D;
-  const real rKb = srb * A / D;
-  const real NEb = Eb / D;
-  Db -= Eb * ES / D;
+  real Ab = srb * rK * iD, Db = -srb * rS * iD;
+  const real rKb = srb * A * iD;
+  const real NEb = Eb * iD;
+  Db -= Eb * ES * iD;
   Ab += NEb * EK;
   const real EKb = NEb * A;
   const real pKb = -NEb * uK;
@@ -107,8 +109,8 @@ __device__ __forceinline__ void hllc_vjp(const real L[5], const real R[5], real 
   Rb[0] += toL ? z : c0; Rb[1] += toL ? z : c1; Rb[4] += toL ? z : c4;
   const real ELb = toL ? cE : z, ERb = toL ? z : cE;
   real SLb = toL ? cS : z, SRb = toL ? z : cS;
-  const real NUb = uSb / den, NPb = pSb / den;
-  const real denb = -(uSb * uS + pSb * pS) / den;
+  const real NUb = uSb * iden, NPb = pSb * iden;
+  const real denb = -(uSb * uS + pSb * pS) * iden;
   real rcRb = NUb * uR + denb, rcLb = NUb * uL + denb;
   Rb[1] += NUb * rcR;
   Lb[1] += NUb * rcL;
@@ -130,11 +132,12 @@ __device__ __forceinline__ void hllc_vjp(const real L[5], const real R[5], real 
   const real cmaxb = SRb - SLb;
   Lb[1] += (uL < uR ? SLb : z) + (uL > uR ? SRb : z);
   Rb[1] += (uL < uR ? z : SLb) + (uL > uR ? z : SRb);
-  const real cLb = cL >= cR ? cmaxb : (real)0, cRb = cL >= cR ? (real)0 : cmaxb;
-  Lb[4] += cLb * gamma / ((real)2 * rL * cL);
-  Lb[0] -= cLb * cL / ((real)2 * rL);
-  Rb[4] += cRb * gamma / ((real)2 * rR * cR);
-  Rb[0] -= cRb * cR / ((real)2 * rR);
+  // cmax = c of one side M: dc/dp = gamma / (2 r c), dc/dr = -c / (2 r)
+  const bool cl = cL >= cR;
+  const real i2r = h / (cl ? rL : rR), i2rc = i2r / cmax;
+  const real b4 = cmaxb * gamma * i2rc, b0 = cmaxb * cmax * i2r;
+  Lb[4] += cl ? b4 : z; Lb[0] -= cl ? b0 : z;
+  Rb[4] += cl ? z : b4; Rb[0] -= cl ? z : b0;
   // E = 0.5 r (u^2 + v^2) + p entho: the second transverse velocity is
   // dropped (RiemannSolvers.h:52-55), so it has no adjoint here
   Lb[0] += ELb * h * (uL * uL + vL * vL); Lb[1] += ELb * rL * uL; Lb[2] += ELb * rL * vL;
@@ -790,37 +793,32 @@ k_adj_faces_tiled(const real* __restrict__ U, const real* __restrict__ v, real s
     }
     return true;
   };
-  // AF_W x AF_R faces, one per thread.  Their contributions are summed in
-  // four passes, one per stencil role: within a pass every face writes a
-  // different window cell, so plain LDS updates suffice (no atomics).  The
-  // tile at the low domain edge also owns face -1/2 of every row, summed in
-  // four more passes.
+  // AF_W x AF_R faces, one per thread, summed into the window with LDS
+  // atomic adds (ds_add_f64 / ds_add_f32).  The tile at the low domain edge
+  // also owns face -1/2 of every row.
   {
     real qb[4][5];
     int wc[4];
     const bool ok = DIR == FV3D_IX ? face(tid / AF_W, tid % AF_W, qb, wc)
                                    : face(tid % AF_W, tid / AF_W, qb, wc);
+    if (ok)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      if (ok)
+      for (int r = 0; r < 4; ++r)
 #pragma unroll
-        for (int n = 0; n < 5; ++n) sa[n][wc[r]] += qb[r][n];
-      __syncthreads();
-    }
+        for (int n = 0; n < 5; ++n) atomicAdd(&sa[n][wc[r]], qb[r][n]);
   }
   if (d0 == 0) {
     real qb[4][5];
     int wc[4];
     const int nx = DIR == FV3D_IX ? AF_R : AF_W;
     const bool ok = tid < nx && face(tid, -1, qb, wc);
+    if (ok)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      if (ok)
+      for (int r = 0; r < 4; ++r)
 #pragma unroll
-        for (int n = 0; n < 5; ++n) sa[n][wc[r]] += qb[r][n];
-      __syncthreads();
-    }
+        for (int n = 0; n < 5; ++n) atomicAdd(&sa[n][wc[r]], qb[r][n]);
   }
+  __syncthreads();
 
   for (int ghosts = 0; ghosts < 2; ++ghosts) {
     if (ghosts) __syncthreads();  // this block's own updates land first
@@ -843,7 +841,7 @@ k_adj_faces_tiled(const real* __restrict__ U, const real* __restrict__ v, real s
   }
 }
 
-constexpr int AD_X = 32, AD_Y = 4, AD_Z = 4;  // diffusion tile
+constexpr int AD_X = 32, AD_Y = 4, AD_Z = 2;  // diffusion tile (256 threads: 3 workgroups per CU)
 
 template <typename real, bool TC, bool VISC>
 __global__ void __launch_bounds__(AD_X * AD_Y * AD_Z)
@@ -883,16 +881,8 @@ k_adj_diffusion_tiled(const real* __restrict__ U, const real* __restrict__ v, re
   const int ti = tid % AD_X, tj = (tid / AD_X) % AD_Y, tk = tid / (AD_X * AD_Y);
   const int i = i0 + ti, j = j0 + tj, k = k0 + tk;
   double gk = 0.0, gm = 0.0;
-  // this cell's contributions to its 3x3x3 neighbourhood, summed below in
-  // one pass per offset (plain LDS updates: within a pass every thread
-  // writes a different window cell)
-  real Sb[3][3][3][3], Tb[3][3][3];
-#pragma unroll
-  for (int e = 0; e < 27; ++e) {
-    (&Tb[0][0][0])[e] = (real)0;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) (&Sb[0][0][0][0])[3 * e + c] = (real)0;
-  }
+  // this cell's contributions to its 3x3x3 neighbourhood go straight into
+  // the window accumulator with LDS atomic adds
   if (i < P.Nx && j < P.Ny && k < P.Nz) {
     const long long cell = i + (long long)j * P.sy + (long long)k * P.sz;
     auto W = [&](int dk, int dj, int di) {  // window index of offset (di-1, dj-1, dk-1)
@@ -929,13 +919,8 @@ k_adj_diffusion_tiled(const real* __restrict__ U, const real* __restrict__ v, re
       for (int o = 0; o < 7; ++o) dE1 += c[o] * sq[3][nb[o]];
       gk = (double)Eb * (double)dE1;
       const real kf = (real)P.kface;
-      Tb[1][1][1] += Eb * kf * c[0];
-      Tb[1][1][0] += Eb * kf * c[1];
-      Tb[1][1][2] += Eb * kf * c[2];
-      Tb[1][0][1] += Eb * kf * c[3];
-      Tb[1][2][1] += Eb * kf * c[4];
-      Tb[0][1][1] += Eb * kf * c[5];
-      Tb[2][1][1] += Eb * kf * c[6];
+#pragma unroll
+      for (int o = 0; o < 7; ++o) atomicAdd(&sa[3][nb[o]], Eb * kf * c[o]);
     }
     if (VISC) {
       // per side: x and y faces once (side 2 of the lower cell, carrying
@@ -947,7 +932,7 @@ k_adj_diffusion_tiled(const real* __restrict__ U, const real* __restrict__ v, re
 #pragma unroll
       for (int c = 0; c < 4; ++c) gi[c] = sdt * v[cell + (c + 1) * P.sf];
       auto Sget = [&](int dk, int dj, int di, int c) -> real { return sq[c][W(dk, dj, di)]; };
-      auto Sacc = [&](int dk, int dj, int di, int c, real x) { Sb[dk][dj][di][c] += x; };
+      auto Sacc = [&](int dk, int dj, int di, int c, real x) { atomicAdd(&sa[c][W(dk, dj, di)], x); };
       double gmu = 0.0;
       auto side = [&](auto blk, int sd, const real gw[4]) {
         constexpr int B = decltype(blk)::value;
@@ -974,20 +959,6 @@ k_adj_diffusion_tiled(const real* __restrict__ U, const real* __restrict__ v, re
       gm = gmu;
     }
   }
-#pragma unroll
-  for (int dk = 0; dk < 3; ++dk)
-#pragma unroll
-    for (int dj = 0; dj < 3; ++dj)
-#pragma unroll
-      for (int di = 0; di < 3; ++di) {
-        const int w = (ti + di) + WX * ((tj + dj) + WY * (tk + dk));
-        if (VISC) {
-#pragma unroll
-          for (int c = 0; c < 3; ++c) sa[c][w] += Sb[dk][dj][di][c];
-        }
-        if (TC) sa[3][w] += Tb[dk][dj][di];
-        __syncthreads();
-      }
   __syncthreads();
   for (int ghosts = 0; ghosts < 2; ++ghosts) {
     if (ghosts) __syncthreads();  // this block's own updates land first

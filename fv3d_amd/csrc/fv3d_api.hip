@@ -248,7 +248,6 @@ int fv3d_get_stage_prim(fv3d_ctx* c, double* Q, int with_ghosts) {
   if (!c || !Q) return fail(c, FV3D_EINVAL, "null argument");
   if (!c->have_stage) return fail(c, FV3D_EINVAL, "no step taken yet");
   HIPCHK(c, hipSetDevice(c->device));
-  if (c->stage_is_inc) return state_prim_to_host(c, c->stage_base, c->B, Q, with_ghosts);
   return state_prim_to_host(c, c->B, nullptr, Q, with_ghosts);
 }
 
@@ -291,21 +290,18 @@ static int launch_step_t(fv3d_ctx* c, hipEvent_t* ev) {
   L.inv_cur = cur;
   real* A = (real*)c->A;
   real* B = (real*)c->B;
-  c->stage_is_inc = false;
   if (kahan(c)) {
-    L.mode = 1;  // STAGE_INC1: K = U* - U^n into B
+    L.mode = 1;  // STAGE_INC1: U* into B
     L.Uin = A; L.Urk = nullptr; L.Uout = B; L.inv_next = nullptr; L.inv_zero = zer;
     if (ev) HIPCHK(c, hipEventRecord(ev[0], c->stream));
     HIPCHK(c, launch_stage<real>(L, c->stream));
     if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
-    L.mode = 2;  // STAGE_INC2: compensated U^{n+1} into C
-    L.Uin = A; L.Kin = B; L.Cio = (real*)c->Cmp; L.Uout = (real*)c->C;
+    L.mode = 2;  // STAGE_INC2: from U*, compensated U^{n+1} into C
+    L.Uin = B; L.Urk = A; L.Cio = (real*)c->Cmp; L.Uout = (real*)c->C;
     L.inv_next = nxt; L.inv_zero = nullptr;
     HIPCHK(c, launch_stage<real>(L, c->stream));
     if (ev) HIPCHK(c, hipEventRecord(ev[2], c->stream));
     std::swap(c->A, c->C);
-    c->stage_is_inc = true;
-    c->stage_base = c->C;
   } else if (c->p.time_stepping == FV3D_TS_RK2) {
     L.Uin = A; L.Urk = nullptr; L.Uout = B; L.inv_next = nullptr; L.inv_zero = zer;
     if (ev) HIPCHK(c, hipEventRecord(ev[0], c->stream));

@@ -376,7 +376,7 @@ static int enqueue_loop(fv3d_batch* b) {
       real* nxt = (s & 1) ? A : C;
       L.mode = 1; L.Uin = cur; L.Urk = nullptr; L.Uout = B; L.inv_next = nullptr; L.inv_zero = ring + ((s + 2) % 3);
       BCHK(b, launch_stage<real>(L, b->stream));
-      L.mode = 2; L.Uin = cur; L.Kin = B; L.Cio = (real*)b->comp + b->off[i]; L.Uout = nxt;
+      L.mode = 2; L.Uin = B; L.Urk = cur; L.Cio = (real*)b->comp + b->off[i]; L.Uout = nxt;
       L.inv_next = ring + ((s + 1) % 3); L.inv_zero = nullptr;
       BCHK(b, launch_stage<real>(L, b->stream));
     } else if (rk2) {

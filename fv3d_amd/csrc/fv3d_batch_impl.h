@@ -22,22 +22,21 @@ namespace fv3d_hip {
 template <typename real, int MODE>
 __device__ __forceinline__ void stage_pointers(const BatchBufs& bb, long long off, long long step,
                                                int stage, int ts, const real*& Uin,
-                                               const real*& Urk, real*& Uout, const real*& Kin,
-                                               real*& Cio) {
+                                               const real*& Urk, real*& Uout, real*& Cio) {
   real* A = (real*)bb.A + off;
   real* B = (real*)bb.B + off;
   Urk = nullptr;
-  Kin = nullptr;
   Cio = nullptr;
   if (MODE == STAGE_INC1 || MODE == STAGE_INC2) {
     real* C = (real*)bb.C + off;
     real* cur = (step & 1) ? C : A;
     real* nxt = (step & 1) ? A : C;
-    Uin = cur;
-    if (MODE == STAGE_INC1) {
+    if (MODE == STAGE_INC1) {  // U* into B
+      Uin = cur;
       Uout = B;
-    } else {
-      Kin = B;
+    } else {  // from U* (B), compensated accumulation onto U^n (cur) into nxt
+      Uin = B;
+      Urk = cur;
       Cio = (real*)bb.comp + off;
       Uout = nxt;
     }
@@ -84,14 +83,14 @@ batch_stage_kernel(const BoxDesc* __restrict__ boxes, const int2* __restrict__ i
     const int2 wi = items[item];
     const BoxDesc* bd = boxes + wi.x;
     const DevParams P = bd->P;
-    const real *Uin, *Urk, *Kin;
+    const real *Uin, *Urk;
     real *Uout, *Cio;
-    stage_pointers<real, MODE>(bb, bd->off, step, stage, ts, Uin, Urk, Uout, Kin, Cio);
+    stage_pointers<real, MODE>(bb, bd->off, step, stage, ts, Uin, Urk, Uout, Cio);
     unsigned long long* ring = bd->ring;
     const unsigned long long* inv_cur = ring + (step % 3);
     unsigned long long* inv_next = last_stage ? ring + ((step + 1) % 3) : nullptr;
     unsigned long long* inv_zero = stage == 0 ? ring + ((step + 2) % 3) : nullptr;
-    stage_tile<real, TY, RECON, RS, TC, VISC, MODE>(Uin, Urk, Uout, Kin, Cio, inv_cur, inv_next,
+    stage_tile<real, TY, RECON, RS, TC, VISC, MODE>(Uin, Urk, Uout, Cio, inv_cur, inv_next,
                                                     inv_zero, bb.health, P, wi.y);
   }
 }

@@ -19,8 +19,6 @@ struct fv3d_ctx {
   void* Cmp = nullptr;  // FP32 compensated RK2: Kahan compensation
   size_t elem = sizeof(double);
   size_t state_bytes = 0;
-  bool stage_is_inc = false;  // B holds the stage-1 increment K (FP32 compensated RK2)
-  void* stage_base = nullptr; // U^n of the last step, when B holds K
   unsigned long long* ring = nullptr;  // 3 slots
   int* health = nullptr;
   double* qbuf = nullptr;  // ghosted SoA primitive staging, allocated on first use

@@ -25,7 +25,6 @@ struct StageLaunch {
   int recon, riemann;
   bool tc, visc;
   int mode = 0;              // STAGE_PLAIN / STAGE_INC1 / STAGE_INC2 (FP32 only)
-  const real* Kin = nullptr;  // stage-1 increment (STAGE_INC2)
   real* Cio = nullptr;        // Kahan compensation, read and written (STAGE_INC2)
   DevParams P;
 };

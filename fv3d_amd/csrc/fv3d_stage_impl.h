@@ -34,22 +34,19 @@ constexpr int NC = WX + 4;  // window columns
 
 // Stage modes.  STAGE_PLAIN: Unew = U + dt L(U) (+ RK2 combine with Urk).
 // FP32 SSP-RK2 with a compensated accumulation uses two further modes:
-// STAGE_INC1 stores the stage-1 increment K = U* - U^n; STAGE_INC2 rebuilds
-// U* = U^n + K on the fly, evaluates U** = U* + dt L(U*) and adds
-// 0.5 (U** - U^n) to U^n with Kahan compensation (array Cio).
+// STAGE_INC1 is the first stage (U* = U^n + dt L(U^n), stored as is, into a
+// buffer of its own); STAGE_INC2 evaluates U** = U* + dt L(U*) from U* and
+// adds 0.5 (U** - U^n) to U^n (Urk) with Kahan compensation (array Cio).
+// The two differ from STAGE_PLAIN in the buffers they take in the batched
+// kernel (the compensated state alternates between two arrays).
 enum { STAGE_PLAIN = 0, STAGE_INC1 = 1, STAGE_INC2 = 2 };
 
-template <typename real, int MODE>
-__device__ __forceinline__ Prim<real> load_prim(const real* __restrict__ U,
-                                                const real* __restrict__ K, long long base,
+template <typename real>
+__device__ __forceinline__ Prim<real> load_prim(const real* __restrict__ U, long long base,
                                                 long long sf, real gm1, bool fx, bool fy,
                                                 bool fz) {
-  real u0 = U[base], u1 = U[base + sf], u2 = U[base + 2 * sf], u3 = U[base + 3 * sf],
-       u4 = U[base + 4 * sf];
-  if (MODE == STAGE_INC2) {
-    u0 = u0 + K[base]; u1 = u1 + K[base + sf]; u2 = u2 + K[base + 2 * sf];
-    u3 = u3 + K[base + 3 * sf]; u4 = u4 + K[base + 4 * sf];
-  }
+  const real u0 = U[base], u1 = U[base + sf], u2 = U[base + 2 * sf], u3 = U[base + 3 * sf],
+             u4 = U[base + 4 * sf];
   Prim<real> q = cons_to_prim(u0, u1, u2, u3, u4, gm1);
   // reflecting ghosts: BoundaryConditions.h:253-258 negates the normal velocity
   if (fx) q.u = -q.u;
@@ -126,7 +123,7 @@ __device__ __forceinline__ void face_flux4(const Prim<real>& a, const Prim<real>
 template <typename real, int TY, int RECON, int RS, bool TC, bool VISC, int MODE,
           bool C91Z = false>
 __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const real* Urk,
-                                           real* Uout, const real* __restrict__ Kin, real* Cio,
+                                           real* Uout, real* Cio,
                                            const unsigned long long* __restrict__ inv_cur,
                                            unsigned long long* inv_next,
                                            unsigned long long* inv_zero, int* health,
@@ -197,13 +194,13 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
     if (C91Z && (kk < 0 || kk >= P.Nz)) return c91_ghost(kk, sxy, fx, fy);
     bool fz;
     const int sk = map_coord(kk, P.Nz, P.bcz, fz);
-    return load_prim<real, MODE>(Uin, Kin, sxy + (long long)sk * P.sz, P.sf, gm1, fx, fy, fz);
+    return load_prim<real>(Uin, sxy + (long long)sk * P.sz, P.sf, gm1, fx, fy, fz);
   };
   auto load_halo = [&](int kk) -> Prim<real> {
     if (C91Z && (kk < 0 || kk >= P.Nz)) return c91_ghost(kk, hxy, hfx, hfy);
     bool fz;
     const int sk = map_coord(kk, P.Nz, P.bcz, fz);
-    return load_prim<real, MODE>(Uin, Kin, hxy + (long long)sk * P.sz, P.sf, gm1, hfx, hfy, fz);
+    return load_prim<real>(Uin, hxy + (long long)sk * P.sz, P.sf, gm1, hfx, hfy, fz);
   };
   // raw conserved values of the halo cell at plane kk, converted later
   auto fetch_halo = [&](int kk, real h[5]) {
@@ -211,10 +208,7 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
     const int sk = map_coord(kk, P.Nz, P.bcz, fz);
     const long long base = hxy + (long long)sk * P.sz;
 #pragma unroll
-    for (int n = 0; n < 5; ++n) {
-      h[n] = Uin[base + n * P.sf];
-      if (MODE == STAGE_INC2) h[n] = h[n] + Kin[base + n * P.sf];
-    }
+    for (int n = 0; n < 5; ++n) h[n] = Uin[base + n * P.sf];
   };
 
   // Registers carried from plane to plane.  Interior waves: cz = z slope at
@@ -293,20 +287,15 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
     const long long cell = (long long)gi + (long long)gj * P.sy + (long long)k * P.sz;
     real ub[5], ur[5], cc[5];
     if (store) {
-      if (MODE == STAGE_INC2) {  // base U* = U^n + K; ur = U^n; cc = compensation
 #pragma unroll
-        for (int n = 0; n < 5; ++n) {
-          ur[n] = Uin[cell + n * P.sf];
-          ub[n] = ur[n] + Kin[cell + n * P.sf];
-          cc[n] = Cio[cell + n * P.sf];
-        }
-      } else {
+      for (int n = 0; n < 5; ++n) ub[n] = Uin[cell + n * P.sf];
+      if (Urk) {
 #pragma unroll
-        for (int n = 0; n < 5; ++n) ub[n] = Uin[cell + n * P.sf];
-        if (Urk) {
+        for (int n = 0; n < 5; ++n) ur[n] = Urk[cell + n * P.sf];
+      }
+      if (MODE == STAGE_INC2) {  // compensation of the U^n accumulation
 #pragma unroll
-          for (int n = 0; n < 5; ++n) ur[n] = Urk[cell + n * P.sf];
-        }
+        for (int n = 0; n < 5; ++n) cc[n] = Cio[cell + n * P.sf];
       }
     }
 
@@ -522,13 +511,7 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
         for (int n = 1; n < 5; ++n) u[n] = u[n] + (vx[n] + vy[n] + vz[n]) * dt;
       }
 
-      if (MODE == STAGE_INC1) {
-        // the stage-1 increment; U^n + K rebuilds this stage's state exactly
-        // whenever the subtraction is exact (Sterbenz), as for any increment
-        // smaller than the state
-#pragma unroll
-        for (int n = 0; n < 5; ++n) Uout[cell + n * P.sf] = u[n] - ub[n];
-      } else if (MODE == STAGE_INC2) {
+      if (MODE == STAGE_INC2) {
         // U^{n+1} = U^n + 0.5 (U** - U^n), Kahan-compensated
 #pragma unroll
         for (int n = 0; n < 5; ++n) {
@@ -591,8 +574,7 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
 
 template <typename real, int TY, int RECON, int RS, bool TC, bool VISC, int MODE, bool C91Z>
 __global__ void __launch_bounds__(64 * (TY + 4))
-stage_kernel(const real* __restrict__ Uin, const real* Urk, real* Uout,
-             const real* __restrict__ Kin, real* Cio,
+stage_kernel(const real* __restrict__ Uin, const real* Urk, real* Uout, real* Cio,
              const unsigned long long* __restrict__ inv_cur, unsigned long long* inv_next,
              unsigned long long* inv_zero, int* health, DevParams P) {
   // Blocks are dealt round-robin to the 8 XCDs (block b -> XCD b%8).  Give
@@ -601,7 +583,7 @@ stage_kernel(const real* __restrict__ Uin, const real* Urk, real* Uout,
   const int nb = gridDim.x, b = blockIdx.x;
   const int q8 = nb >> 3, r8 = nb & 7, xcd = b & 7, loc = b >> 3;
   const int tile = xcd * q8 + (xcd < r8 ? xcd : r8) + loc;
-  stage_tile<real, TY, RECON, RS, TC, VISC, MODE, C91Z>(Uin, Urk, Uout, Kin, Cio, inv_cur, inv_next,
+  stage_tile<real, TY, RECON, RS, TC, VISC, MODE, C91Z>(Uin, Urk, Uout, Cio, inv_cur, inv_next,
                                                   inv_zero, health, P, tile);
 }
 
@@ -613,7 +595,7 @@ static hipError_t launch_t(const StageLaunch<real>& L, hipStream_t s) {
   DevParams P = L.P;
   const int blocks = P.tiles_x * P.tiles_y * P.tiles_z;
   hipLaunchKernelGGL((stage_kernel<real, TY, RECON, RS, TC, VISC, MODE, C91Z>), dim3(blocks),
-                     dim3(64 * (TY + 4)), 0, s, L.Uin, L.Urk, L.Uout, L.Kin, L.Cio, L.inv_cur,
+                     dim3(64 * (TY + 4)), 0, s, L.Uin, L.Urk, L.Uout, L.Cio, L.inv_cur,
                      L.inv_next, L.inv_zero, L.health, P);
   return hipGetLastError();
 }
@@ -631,9 +613,9 @@ static hipError_t launch_phys(const StageLaunch<real>& L, hipStream_t s) {
 // domain column, from the stage's input state, as primitives.  The
 // reference multiplies by the gravity flag where g belongs (its Q17); so
 // does this.
-template <typename real, int MODE>
+template <typename real>
 __global__ void __launch_bounds__(256)
-k_c91_ghosts(const real* __restrict__ U, const real* __restrict__ K, real* g, DevParams P) {
+k_c91_ghosts(const real* __restrict__ U, real* g, DevParams P) {
   const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (t >= 4 * P.sz) return;
   const int plane = (int)(t / P.sz);
@@ -642,8 +624,8 @@ k_c91_ghosts(const real* __restrict__ U, const real* __restrict__ K, real* g, De
   const int kref = kk < 0 ? 0 : P.Nz - 1;
   const int ksym = kk < 0 ? -kk - 1 : 2 * P.Nz - kk - 1;
   const real gm1 = (real)P.gm1;
-  const Prim<real> qr = load_prim<real, MODE>(U, K, xy + (long long)kref * P.sz, P.sf, gm1, false, false, false);
-  const Prim<real> qs = load_prim<real, MODE>(U, K, xy + (long long)ksym * P.sz, P.sf, gm1, false, false, false);
+  const Prim<real> qr = load_prim<real>(U, xy + (long long)kref * P.sz, P.sf, gm1, false, false, false);
+  const Prim<real> qs = load_prim<real>(U, xy + (long long)ksym * P.sz, P.sf, gm1, false, false, false);
   const real z = (real)P.zmin + ((real)kk + (real)0.5) * (real)P.dz;
   const real grav = P.gravity ? (real)1 : (real)0;
   real res[5];
@@ -680,8 +662,8 @@ template <typename real, int MODE>
 static hipError_t launch_mode(const StageLaunch<real>& L, hipStream_t s) {
   if (L.P.bcz == FV3D_BC_C91) {
     const long long n = 4 * L.P.sz;
-    hipLaunchKernelGGL((k_c91_ghosts<real, MODE>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
-                       L.Uin, L.Kin, (real*)L.P.c91g, L.P);
+    hipLaunchKernelGGL((k_c91_ghosts<real>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                       L.Uin, (real*)L.P.c91g, L.P);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
