@@ -249,10 +249,16 @@ __device__ __forceinline__ void slope_sel(const real a[5], const real b[5], cons
 template <typename real>
 __device__ __forceinline__ void slope_vjp(const int sel[5], const real sb[5], real ab[5],
                                           real bb[5], real cb[5]) {
+  // selects on values: branches that update different arrays would put
+  // the arrays in scratch memory
+  const real z = (real)0;
 #pragma unroll
   for (int n = 0; n < 5; ++n) {
-    if (sel[n] > 0) { bb[n] += sb[n]; ab[n] -= sb[n]; }
-    else if (sel[n] < 0) { cb[n] += sb[n]; bb[n] -= sb[n]; }
+    const real x = sb[n];
+    const bool bw = sel[n] > 0, fw = sel[n] < 0;
+    ab[n] -= bw ? x : z;
+    bb[n] += bw ? x : (fw ? -x : z);
+    cb[n] += fw ? x : z;
   }
 }
 
@@ -841,6 +847,127 @@ k_adj_faces_tiled(const real* __restrict__ U, const real* __restrict__ v, real s
   }
 }
 
+// Faces along y or z, marched by one thread per (x, transverse) line and
+// chunk of faces.  The four-cell stencil window, the slope of its second
+// cell and the adjoints of the four cells slide along the line in
+// registers: a face costs one new cell (its conversion and its loads of U
+// and v) and one minmod selection, and a cell leaves the registers once per
+// chunk, with one global atomic add per value.  Lanes run along x, so every
+// load and atomic is coalesced.  Faces m+1/2 with m in [m0, m1), m0 =
+// -1 + chunk*F; they reach cells m0-1 .. m1+1.
+constexpr int AL_T = 4;  // lines (transverse coordinates) per 256-thread workgroup
+
+template <typename real, int DIR, int RECON, int RS>
+__global__ void __launch_bounds__(64 * AL_T)
+k_adj_faces_line(const real* __restrict__ U, const real* __restrict__ v, real sigma, real* Qb,
+                 double dt, DevParams P, int F) {
+  static_assert(DIR == FV3D_IY || DIR == FV3D_IZ, "x faces use the tiled kernel");
+  constexpr bool ZORD = DIR == FV3D_IZ;
+  const int Nd = DIR == FV3D_IY ? P.Ny : P.Nz;
+  const int Nt = DIR == FV3D_IY ? P.Nz : P.Ny;
+  const int bc = DIR == FV3D_IY ? P.bcy : P.bcz;
+  const long long stride = DIR == FV3D_IY ? P.sy : P.sz;
+  const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int t = blockIdx.y * AL_T + (threadIdx.x >> 6);
+  const int m0 = -1 + (int)blockIdx.z * F;
+  const int m1 = min(m0 + F, Nd);
+  if (i >= P.Nx || t >= Nt || m0 >= m1) return;
+  const long long base = i + (long long)t * (DIR == FV3D_IY ? P.sz : P.sy);
+  const real gm1 = (real)P.gm1;
+  const real coef = (real)((double)sigma * dt / (DIR == FV3D_IY ? P.dy : P.dz));
+
+  real q[4][5];  // primitives of cells m-1 .. m+2 (ghosts mirrored)
+  long long src[4];
+  bool flip[4];
+  auto load = [&](int o, int cm) {
+    bool f;
+    const int sidx = map_coord(cm, Nd, bc, f);
+    src[o] = base + (long long)sidx * stride;
+    flip[o] = f;
+    load_prim_arr(U, src[o], P.sf, gm1, q[o]);
+    if (f) q[o][1 + DIR] = -q[o][1 + DIR];
+  };
+  real acc[4][5];
+  auto flush = [&](int o) {
+#pragma unroll
+    for (int n = 0; n < 5; ++n) {
+      const real x = (flip[o] && n == 1 + DIR) ? -acc[o][n] : acc[o][n];
+      atomic_add(&Qb[src[o] + n * P.sf], x);
+    }
+  };
+  load(0, m0 - 1);
+  load(1, m0);
+  load(2, m0 + 1);
+#pragma unroll
+  for (int o = 0; o < 4; ++o)
+#pragma unroll
+    for (int n = 0; n < 5; ++n) acc[o][n] = (real)0;
+  real vb[5];  // v of cell m (zero outside the domain)
+#pragma unroll
+  for (int n = 0; n < 5; ++n) vb[n] = m0 >= 0 ? v[src[1] + n * P.sf] : (real)0;
+  real sb[5] = {0, 0, 0, 0, 0};  // slope of cell m, carried from the previous face
+  int selb[5] = {0, 0, 0, 0, 0};
+  if (RECON == FV3D_PLM) slope_sel<real, ZORD>(q[0], q[1], q[2], sb, selb);
+
+  for (int m = m0; m < m1; ++m) {
+    load(3, m + 2);
+    real Fb[5], vc[5];
+#pragma unroll
+    for (int n = 0; n < 5; ++n) {
+      vc[n] = m + 1 < Nd ? v[src[2] + n * P.sf] : (real)0;
+      Fb[n] = coef * (vc[n] - vb[n]);
+    }
+    real qL[5], qR[5], sc[5] = {0, 0, 0, 0, 0};
+    int selc[5] = {0, 0, 0, 0, 0};
+    if (RECON == FV3D_PLM) {
+      slope_sel<real, ZORD>(q[1], q[2], q[3], sc, selc);
+#pragma unroll
+      for (int n = 0; n < 5; ++n) {
+        qL[n] = q[1][n] + sb[n] * (real)0.5;
+        qR[n] = q[2][n] - sc[n] * (real)0.5;
+      }
+    } else {
+#pragma unroll
+      for (int n = 0; n < 5; ++n) { qL[n] = q[1][n]; qR[n] = q[2][n]; }
+    }
+    swap_dir<DIR>(qL);
+    swap_dir<DIR>(qR);
+    swap_dir<DIR>(Fb);
+    real Lb[5] = {0, 0, 0, 0, 0}, Rb[5] = {0, 0, 0, 0, 0};
+    if (RS == FV3D_HLL) hll_vjp(qL, qR, (real)P.gamma, gm1, Fb, Lb, Rb);
+    else hllc_vjp(qL, qR, (real)P.gamma, (real)P.entho, Fb, Lb, Rb);
+    swap_dir<DIR>(Lb);
+    swap_dir<DIR>(Rb);
+#pragma unroll
+    for (int n = 0; n < 5; ++n) { acc[1][n] += Lb[n]; acc[2][n] += Rb[n]; }
+    if (RECON == FV3D_PLM) {
+      real ssb[5], ssc[5];
+#pragma unroll
+      for (int n = 0; n < 5; ++n) { ssb[n] = (real)0.5 * Lb[n]; ssc[n] = (real)-0.5 * Rb[n]; }
+      slope_vjp(selb, ssb, acc[0], acc[1], acc[2]);
+      slope_vjp(selc, ssc, acc[1], acc[2], acc[3]);
+    }
+    flush(0);  // cell m-1: no later face of the chunk reaches it
+#pragma unroll
+    for (int o = 0; o < 3; ++o) {
+      src[o] = src[o + 1];
+      flip[o] = flip[o + 1];
+#pragma unroll
+      for (int n = 0; n < 5; ++n) { q[o][n] = q[o + 1][n]; acc[o][n] = acc[o + 1][n]; }
+    }
+#pragma unroll
+    for (int n = 0; n < 5; ++n) {
+      acc[3][n] = (real)0;
+      vb[n] = vc[n];
+      sb[n] = sc[n];
+      selb[n] = selc[n];
+    }
+  }
+  flush(0);
+  flush(1);
+  flush(2);
+}
+
 constexpr int AD_X = 32, AD_Y = 4, AD_Z = 2;  // diffusion tile (256 threads: 3 workgroups per CU)
 
 template <typename real, bool TC, bool VISC>
@@ -1069,6 +1196,15 @@ static hipError_t adj_faces_dir(const AdjLaunch<real>& L, hipStream_t s) {
     const int bs = 256;
     hipLaunchKernelGGL((k_adj_faces<real, DIR, RECON, RS>), dim3((unsigned)((nf + bs - 1) / bs)),
                        dim3(bs), 0, s, L.U, L.v, L.sigma, L.Qb, L.dt, P);
+    return hipGetLastError();
+  }
+  if (DIR != FV3D_IX && !L.colored) {
+    // chunks of about 16 faces along the line
+    const int Nd = DIR == FV3D_IY ? P.Ny : P.Nz, Nt = DIR == FV3D_IY ? P.Nz : P.Ny;
+    const int nch = (Nd + 1 + 15) / 16, F = (Nd + 1 + nch - 1) / nch;
+    const dim3 g((P.Nx + 63) / 64, (Nt + AL_T - 1) / AL_T, (Nd + 1 + F - 1) / F);
+    hipLaunchKernelGGL((k_adj_faces_line<real, DIR == FV3D_IX ? FV3D_IY : DIR, RECON, RS>), g,
+                       dim3(64 * AL_T), 0, s, L.U, L.v, L.sigma, L.Qb, L.dt, P, F);
     return hipGetLastError();
   }
   dim3 grid;
