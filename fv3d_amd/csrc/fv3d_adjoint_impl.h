@@ -28,6 +28,7 @@
 // so sums may associate differently from run to run.
 #pragma once
 
+#include <algorithm>
 #include <type_traits>
 
 #include "fv3d_device.h"
@@ -968,6 +969,92 @@ k_adj_faces_line(const real* __restrict__ U, const real* __restrict__ v, real si
   flush(2);
 }
 
+// The diffusion adjoint of one cell (i, j, k): the transposes of its
+// conduction update (ThermalConduction.h:47-95) and of its viscous sides
+// (Viscosity.h:28-153; x and y faces once, as side 2 of the lower cell with
+// both cells' adjoints, side 1 only at the low domain edge; z sides per
+// cell), and the cell's dS/dkappa, dS/dmu dotted with sigma*v.  S(dk, dj,
+// di, c) reads the window value c (u, v, w, T) at offset (di-1, dj-1, dk-1),
+// mirrored for ghosts; A(dk, dj, di, c, x) adds x to its adjoint.
+template <typename real, bool TC, bool VISC, typename GET, typename ADD>
+__device__ __forceinline__ void diffusion_cell_adjoint(const real* __restrict__ v, real sigma,
+                                                       double dt, const DevParams& P, int i,
+                                                       int j, int k, long long cell, GET&& S,
+                                                       ADD&& A, double& gk, double& gm) {
+  const real rdt = (real)dt;
+  if (TC) {
+    const real Eb = sigma * v[cell + 4 * P.sf];
+    // stencil offsets (dk, dj, di): centre, x-, x+, y-, y+, z-, z+
+    constexpr int ok[7] = {1, 1, 1, 1, 1, 0, 2}, oj[7] = {1, 1, 1, 0, 2, 1, 1},
+                  oi[7] = {1, 0, 2, 1, 1, 1, 1};
+    const real ax = rdt / (real)(P.dx * P.dx), ay = rdt / (real)(P.dy * P.dy),
+               az = rdt / (real)(P.dz * P.dz);
+    real c[7] = {-(real)2 * (ax + ay + az), ax, ax, ay, ay, az, az};
+    real fixedE = 0;
+    if (k == 0 && P.bctc_zmin != FV3D_BCTC_NONE) {
+      c[0] += az; c[5] -= az;
+      if (P.bctc_zmin == FV3D_BCTC_FIXED_TEMPERATURE) {
+        c[0] -= (real)2 * az;
+        fixedE += (real)2 * az * (real)P.bctc_zmin_value;
+      } else if (P.bctc_zmin == FV3D_BCTC_FIXED_GRADIENT) {
+        fixedE -= rdt / (real)P.dz * (real)P.bctc_zmin_value;
+      }
+    }
+    if (k == P.Nz - 1 && P.bctc_zmax != FV3D_BCTC_NONE) {
+      c[0] += az; c[6] -= az;
+      if (P.bctc_zmax == FV3D_BCTC_FIXED_TEMPERATURE) {
+        c[0] -= (real)2 * az;
+        fixedE += (real)2 * az * (real)P.bctc_zmax_value;
+      } else if (P.bctc_zmax == FV3D_BCTC_FIXED_GRADIENT) {
+        fixedE += rdt / (real)P.dz * (real)P.bctc_zmax_value;
+      }
+    }
+    real dE1 = fixedE;
+#pragma unroll
+    for (int o = 0; o < 7; ++o) dE1 += c[o] * S(ok[o], oj[o], oi[o], 3);
+    gk = (double)Eb * (double)dE1;
+    const real kf = (real)P.kface;
+#pragma unroll
+    for (int o = 0; o < 7; ++o) A(ok[o], oj[o], oi[o], 3, Eb * kf * c[o]);
+  }
+  if (VISC) {
+    // per side: x and y faces once (side 2 of the lower cell, carrying
+    // both cells' adjoints; side 1 only at the low domain edge), z sides
+    // per cell
+    const real oodx = (real)1 / (real)P.dx, oody = (real)1 / (real)P.dy, oodz = (real)1 / (real)P.dz;
+    const real sdt = sigma * rdt, mu = (real)P.mu;
+    real gi[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) gi[c] = sdt * v[cell + (c + 1) * P.sf];
+    auto Sget = [&](int dk, int dj, int di, int c) -> real { return S(dk, dj, di, c); };
+    auto Sacc = [&](int dk, int dj, int di, int c, real x) { A(dk, dj, di, c, x); };
+    double gmu = 0.0;
+    auto side = [&](auto blk, int sd, const real gw[4]) {
+      constexpr int B = decltype(blk)::value;
+      real out[4] = {0, 0, 0, 0}, gm4[4];
+      visc_side<real, false, B>(Sget, sd, oodx, oody, oodz, gw, out, Sacc);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) { gmu += (double)gw[c] * (double)out[c]; gm4[c] = gw[c] * mu; }
+      visc_side<real, true, B>(Sget, sd, oodx, oody, oodz, gm4, out, Sacc);
+    };
+    using BX = std::integral_constant<int, 0>;
+    using BY = std::integral_constant<int, 1>;
+    using BZ = std::integral_constant<int, 2>;
+    real gf[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) gf[c] = gi[c] - (i + 1 < P.Nx ? sdt * v[cell + 1 + (c + 1) * P.sf] : (real)0);
+    side(BX{}, 2, gf);
+    if (i == 0) side(BX{}, 1, gi);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) gf[c] = gi[c] - (j + 1 < P.Ny ? sdt * v[cell + P.sy + (c + 1) * P.sf] : (real)0);
+    side(BY{}, 2, gf);
+    if (j == 0) side(BY{}, 1, gi);
+    side(BZ{}, 1, gi);
+    side(BZ{}, 2, gi);
+    gm = gmu;
+  }
+}
+
 constexpr int AD_X = 32, AD_Y = 4, AD_Z = 2;  // diffusion tile (256 threads: 3 workgroups per CU)
 
 template <typename real, bool TC, bool VISC>
@@ -1015,76 +1102,10 @@ k_adj_diffusion_tiled(const real* __restrict__ U, const real* __restrict__ v, re
     auto W = [&](int dk, int dj, int di) {  // window index of offset (di-1, dj-1, dk-1)
       return (ti + di) + WX * ((tj + dj) + WY * (tk + dk));
     };
-    const real rdt = (real)dt;
-    if (TC) {
-      const real Eb = sigma * v[cell + 4 * P.sf];
-      const int nb[7] = {W(1, 1, 1), W(1, 1, 0), W(1, 1, 2), W(1, 0, 1), W(1, 2, 1), W(0, 1, 1), W(2, 1, 1)};
-      const real ax = rdt / (real)(P.dx * P.dx), ay = rdt / (real)(P.dy * P.dy),
-                 az = rdt / (real)(P.dz * P.dz);
-      real c[7] = {-(real)2 * (ax + ay + az), ax, ax, ay, ay, az, az};
-      real fixedE = 0;
-      if (k == 0 && P.bctc_zmin != FV3D_BCTC_NONE) {
-        c[0] += az; c[5] -= az;
-        if (P.bctc_zmin == FV3D_BCTC_FIXED_TEMPERATURE) {
-          c[0] -= (real)2 * az;
-          fixedE += (real)2 * az * (real)P.bctc_zmin_value;
-        } else if (P.bctc_zmin == FV3D_BCTC_FIXED_GRADIENT) {
-          fixedE -= rdt / (real)P.dz * (real)P.bctc_zmin_value;
-        }
-      }
-      if (k == P.Nz - 1 && P.bctc_zmax != FV3D_BCTC_NONE) {
-        c[0] += az; c[6] -= az;
-        if (P.bctc_zmax == FV3D_BCTC_FIXED_TEMPERATURE) {
-          c[0] -= (real)2 * az;
-          fixedE += (real)2 * az * (real)P.bctc_zmax_value;
-        } else if (P.bctc_zmax == FV3D_BCTC_FIXED_GRADIENT) {
-          fixedE += rdt / (real)P.dz * (real)P.bctc_zmax_value;
-        }
-      }
-      real dE1 = fixedE;
-#pragma unroll
-      for (int o = 0; o < 7; ++o) dE1 += c[o] * sq[3][nb[o]];
-      gk = (double)Eb * (double)dE1;
-      const real kf = (real)P.kface;
-#pragma unroll
-      for (int o = 0; o < 7; ++o) atomicAdd(&sa[3][nb[o]], Eb * kf * c[o]);
-    }
-    if (VISC) {
-      // per side: x and y faces once (side 2 of the lower cell, carrying
-      // both cells' adjoints; side 1 only at the low domain edge), z sides
-      // per cell
-      const real oodx = (real)1 / (real)P.dx, oody = (real)1 / (real)P.dy, oodz = (real)1 / (real)P.dz;
-      const real sdt = sigma * rdt, mu = (real)P.mu;
-      real gi[4];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) gi[c] = sdt * v[cell + (c + 1) * P.sf];
-      auto Sget = [&](int dk, int dj, int di, int c) -> real { return sq[c][W(dk, dj, di)]; };
-      auto Sacc = [&](int dk, int dj, int di, int c, real x) { atomicAdd(&sa[c][W(dk, dj, di)], x); };
-      double gmu = 0.0;
-      auto side = [&](auto blk, int sd, const real gw[4]) {
-        constexpr int B = decltype(blk)::value;
-        real out[4] = {0, 0, 0, 0}, gm4[4];
-        visc_side<real, false, B>(Sget, sd, oodx, oody, oodz, gw, out, Sacc);
-#pragma unroll
-        for (int c = 0; c < 4; ++c) { gmu += (double)gw[c] * (double)out[c]; gm4[c] = gw[c] * mu; }
-        visc_side<real, true, B>(Sget, sd, oodx, oody, oodz, gm4, out, Sacc);
-      };
-      using BX = std::integral_constant<int, 0>;
-      using BY = std::integral_constant<int, 1>;
-      using BZ = std::integral_constant<int, 2>;
-      real gf[4];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) gf[c] = gi[c] - (i + 1 < P.Nx ? sdt * v[cell + 1 + (c + 1) * P.sf] : (real)0);
-      side(BX{}, 2, gf);
-      if (i == 0) side(BX{}, 1, gi);
-#pragma unroll
-      for (int c = 0; c < 4; ++c) gf[c] = gi[c] - (j + 1 < P.Ny ? sdt * v[cell + P.sy + (c + 1) * P.sf] : (real)0);
-      side(BY{}, 2, gf);
-      if (j == 0) side(BY{}, 1, gi);
-      side(BZ{}, 1, gi);
-      side(BZ{}, 2, gi);
-      gm = gmu;
-    }
+    diffusion_cell_adjoint<real, TC, VISC>(
+        v, sigma, dt, P, i, j, k, cell,
+        [&](int dk, int dj, int di, int c) -> real { return sq[c][W(dk, dj, di)]; },
+        [&](int dk, int dj, int di, int c, real x) { atomicAdd(&sa[c][W(dk, dj, di)], x); }, gk, gm);
   }
   __syncthreads();
   for (int ghosts = 0; ghosts < 2; ++ghosts) {
@@ -1126,6 +1147,108 @@ k_adj_diffusion_tiled(const real* __restrict__ U, const real* __restrict__ v, re
     double a = 0.0, b = 0.0;
     for (int w = 0; w < AD_X * AD_Y * AD_Z / 64; ++w) { a += sred[0][w]; b += sred[1][w]; }
     const long long t = (i0 / AD_X) + (long long)ntiles.x * ((j0 / AD_Y) + (long long)ntiles.y * (k0 / AD_Z));
+    grad[2 + 2 * t] = a;
+    grad[3 + 2 * t] = b;
+  }
+}
+
+// The diffusion adjoint marched along z: a workgroup owns a 64 x 4 column
+// of a z chunk and keeps three planes of its (66 x 6) window, primitives
+// and adjoints, in LDS rings.  Each window cell is converted once per chunk
+// (plus two halo planes), and a plane's adjoints leave LDS once, with one
+// global atomic add per value, when the march has passed it.
+constexpr int AZ_X = 64, AZ_Y = 4;
+
+template <typename real, bool TC, bool VISC>
+__global__ void __launch_bounds__(AZ_X * AZ_Y)
+k_adj_diffusion_march(const real* __restrict__ U, const real* __restrict__ v, real sigma,
+                      real* Qb, double dt, DevParams P, double* grad, int nzc) {
+  constexpr int WX = AZ_X + 2, WY = AZ_Y + 2, NWP = WX * WY;
+  __shared__ real sq[3][4][NWP];  // u, v, w, T of three window planes (ghost-mapped)
+  __shared__ real sa[3][4][NWP];  // their adjoints
+  const int tid = threadIdx.x;
+  const int i0 = blockIdx.x * AZ_X, j0 = blockIdx.y * AZ_Y, k0 = blockIdx.z * nzc;
+  const int k1 = min(k0 + nzc, P.Nz);
+  const real gm1 = (real)P.gm1;
+  auto slot = [](int kk) { return (kk + 3) % 3; };  // kk >= -1
+  auto load_plane = [&](int kk) {
+    const int sl = slot(kk);
+    bool fz;
+    const int sk = map_coord(kk, P.Nz, P.bcz, fz);
+    for (int w = tid; w < NWP; w += AZ_X * AZ_Y) {
+      bool fx, fy;
+      const int si = map_coord(i0 - 1 + w % WX, P.Nx, P.bcx, fx);
+      const int sj = map_coord(j0 - 1 + w / WX, P.Ny, P.bcy, fy);
+      real q[5];
+      load_prim_arr(U, si + (long long)sj * P.sy + (long long)sk * P.sz, P.sf, gm1, q);
+      sq[sl][0][w] = fx ? -q[1] : q[1];
+      sq[sl][1][w] = fy ? -q[2] : q[2];
+      sq[sl][2][w] = fz ? -q[3] : q[3];
+      sq[sl][3][w] = q[4] / q[0];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) sa[sl][c][w] = 0;
+    }
+  };
+  auto flush_plane = [&](int kk) {
+    if (kk < -1 || kk > P.Nz) return;
+    const int sl = slot(kk);
+    bool fz;
+    const int sk = map_coord(kk, P.Nz, P.bcz, fz);
+    for (int w = tid; w < NWP; w += AZ_X * AZ_Y) {
+      const int wi = i0 - 1 + w % WX, wj = j0 - 1 + w / WX;
+      if (wi > P.Nx || wj > P.Ny) continue;
+      bool fx, fy;
+      const int si = map_coord(wi, P.Nx, P.bcx, fx), sj = map_coord(wj, P.Ny, P.bcy, fy);
+      const long long src = si + (long long)sj * P.sy + (long long)sk * P.sz;
+      if (VISC) {
+        atomic_add(&Qb[src + 1 * P.sf], fx ? -sa[sl][0][w] : sa[sl][0][w]);
+        atomic_add(&Qb[src + 2 * P.sf], fy ? -sa[sl][1][w] : sa[sl][1][w]);
+        atomic_add(&Qb[src + 3 * P.sf], fz ? -sa[sl][2][w] : sa[sl][2][w]);
+      }
+      if (TC) atomic_add(&Qb[src + 5 * P.sf], sa[sl][3][w]);
+    }
+  };
+
+  const int ti = tid % AZ_X, tj = tid / AZ_X;
+  const int i = i0 + ti, j = j0 + tj;
+  double gk = 0.0, gm = 0.0;
+  load_plane(k0 - 1);
+  load_plane(k0);
+  for (int k = k0; k < k1; ++k) {
+    load_plane(k + 1);
+    __syncthreads();
+    if (i < P.Nx && j < P.Ny) {
+      const long long cell = i + (long long)j * P.sy + (long long)k * P.sz;
+      const int sl[3] = {slot(k - 1), slot(k), slot(k + 1)};
+      auto W = [&](int dj, int di) { return (ti + di) + WX * (tj + dj); };
+      double gkc = 0.0, gmc = 0.0;
+      diffusion_cell_adjoint<real, TC, VISC>(
+          v, sigma, dt, P, i, j, k, cell,
+          [&](int dk, int dj, int di, int c) -> real { return sq[sl[dk]][c][W(dj, di)]; },
+          [&](int dk, int dj, int di, int c, real x) { atomicAdd(&sa[sl[dk]][c][W(dj, di)], x); },
+          gkc, gmc);
+      gk += gkc;
+      gm += gmc;
+    }
+    __syncthreads();
+    flush_plane(k - 1);  // no later cell of the chunk reaches plane k-1
+    __syncthreads();     // before the next plane overwrites its slot
+  }
+  flush_plane(k1 - 1);
+  flush_plane(k1);
+  // parameter gradients: one partial per workgroup
+  __shared__ double sred[2][AZ_X * AZ_Y / 64];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    gk += __shfl_xor(gk, off);
+    gm += __shfl_xor(gm, off);
+  }
+  if ((tid & 63) == 0) { sred[0][tid >> 6] = gk; sred[1][tid >> 6] = gm; }
+  __syncthreads();
+  if (tid == 0) {
+    double a = 0.0, b = 0.0;
+    for (int w = 0; w < AZ_X * AZ_Y / 64; ++w) { a += sred[0][w]; b += sred[1][w]; }
+    const long long t = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
     grad[2 + 2 * t] = a;
     grad[3 + 2 * t] = b;
   }
@@ -1245,10 +1368,22 @@ static void adj_diffusion(const AdjLaunch<real>& L, hipStream_t s) {
                        dim3(bs), 0, s, L.U, L.v, L.sigma, L.Qb, L.dt, P, L.grad);
     return;
   }
+  if (!L.colored) {
+    // z chunks of ~16 planes, at least ~2 workgroups per CU in all
+    const int tx = (P.Nx + AZ_X - 1) / AZ_X, ty = (P.Ny + AZ_Y - 1) / AZ_Y;
+    int nch = std::max(1, std::min((P.Nz + 7) / 8, (512 + tx * ty - 1) / (tx * ty)));
+    const int nzc = (P.Nz + nch - 1) / nch;
+    nch = (P.Nz + nzc - 1) / nzc;
+    hipLaunchKernelGGL((k_adj_diffusion_march<real, TC, VISC>), dim3(tx, ty, nch), dim3(AZ_X * AZ_Y),
+                       0, s, L.U, L.v, L.sigma, L.Qb, L.dt, P, L.grad, nzc);
+    hipLaunchKernelGGL((k_adj_reduce<real>), dim3(1), dim3(256), 0, s, L.grad,
+                       (long long)tx * ty * nch);
+    return;
+  }
   const dim3 grid((P.Nx + AD_X - 1) / AD_X, (P.Ny + AD_Y - 1) / AD_Y, (P.Nz + AD_Z - 1) / AD_Z);
   const int3 nt = make_int3(grid.x, grid.y, grid.z);
   const long long ntl = (long long)nt.x * nt.y * nt.z;
-  if (!L.colored || !L.colored_diffusion) {
+  if (!L.colored_diffusion) {
     hipLaunchKernelGGL((k_adj_diffusion_tiled<real, TC, VISC>), grid, dim3(AD_X * AD_Y * AD_Z), 0, s,
                        L.U, L.v, L.sigma, L.Qb, L.dt, P, L.grad, make_int3(-1, -1, -1), nt);
     hipLaunchKernelGGL((k_adj_reduce<real>), dim3(1), dim3(256), 0, s, L.grad, ntl);
