@@ -34,8 +34,12 @@ SOURCES = ["fv3d_stage.hip", "fv3d_stage_f32.hip", "fv3d_batch.hip", "fv3d_batch
            "fv3d_adjoint_f32.hip", "fv3d_adjoint_api.hip"]
 # per-source extra flags.  The FP32 kernels flush FP32 denormals (FP64 keeps
 # them): state values sit far above 1e-38, and without denormal support the
-# FP32 square root and division need no range scaling.
-F32 = ["-fno-hip-fp32-correctly-rounded-divide-sqrt", "-fgpu-flush-denormals-to-zero"]
+# FP32 square root and division need no range scaling.  They also contract
+# a*b + c within one expression into an FMA (-ffp-contract=on, decided per
+# source expression, so the batched and single-box FP32 kernels stay
+# bit-equal); the FP64 translation units keep -ffp-contract=off.
+F32 = ["-fno-hip-fp32-correctly-rounded-divide-sqrt", "-fgpu-flush-denormals-to-zero",
+       "-ffp-contract=on"]
 EXTRA = {
     "fv3d_stage_f32.hip": F32,
     "fv3d_batch_f32.hip": F32,

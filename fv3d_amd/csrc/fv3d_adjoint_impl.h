@@ -404,10 +404,9 @@ __device__ __forceinline__ void visc_side(SF&& S, const int s, real oodx, real o
     const real txx = f43 * dudx - f23 * (dvdy + dwdz);
     const real txy = dvdx + dudy;
     const real txz = dwdx + dudz;
-    if (!REV) {
-      out[0] += sgn * txx; out[1] += sgn * txy; out[2] += sgn * txz;
-      out[3] += sgn * (txx * qiu + txy * qiv + txz * qiw);
-    } else {
+    out[0] += sgn * txx; out[1] += sgn * txy; out[2] += sgn * txz;
+    out[3] += sgn * (txx * qiu + txy * qiv + txz * qiw);
+    if (REV) {
       const real txxb = sgn * (g[0] + g[3] * qiu), txyb = sgn * (g[1] + g[3] * qiv),
                  txzb = sgn * (g[2] + g[3] * qiw);
       const real qiub = sgn * g[3] * txx * h, qivb = sgn * g[3] * txy * h, qiwb = sgn * g[3] * txz * h;
@@ -442,10 +441,9 @@ __device__ __forceinline__ void visc_side(SF&& S, const int s, real oodx, real o
     const real tyy = f43 * dvdy - f23 * (dudx + dwdz);
     const real txy = dvdx + dudy;
     const real tyz = dvdz + dwdy;
-    if (!REV) {
-      out[0] += sgn * txy; out[1] += sgn * tyy; out[2] += sgn * tyz;
-      out[3] += sgn * (txy * qiu + tyy * qiv + tyz * qiw);
-    } else {
+    out[0] += sgn * txy; out[1] += sgn * tyy; out[2] += sgn * tyz;
+    out[3] += sgn * (txy * qiu + tyy * qiv + tyz * qiw);
+    if (REV) {
       const real txyb = sgn * (g[0] + g[3] * qiu), tyyb = sgn * (g[1] + g[3] * qiv),
                  tyzb = sgn * (g[2] + g[3] * qiw);
       const real qiub = sgn * g[3] * txy * h, qivb = sgn * g[3] * tyy * h, qiwb = sgn * g[3] * tyz * h;
@@ -481,10 +479,9 @@ __device__ __forceinline__ void visc_side(SF&& S, const int s, real oodx, real o
     const real tzz = f43 * dwdz - f23 * (dudx + dvdy);
     const real txz = dudz + dwdx;
     const real tyz = dvdz + dwdy;
-    if (!REV) {
-      out[0] += sgn * txz; out[1] += sgn * tyz; out[2] += sgn * tzz;
-      out[3] += sgn * (txz * qiu + tyz * qiv + tzz * qiw);
-    } else {
+    out[0] += sgn * txz; out[1] += sgn * tyz; out[2] += sgn * tzz;
+    out[3] += sgn * (txz * qiu + tyz * qiv + tzz * qiw);
+    if (REV) {
       const real txzb = sgn * (g[0] + g[3] * qiu), tyzb = sgn * (g[1] + g[3] * qiv),
                  tzzb = sgn * (g[2] + g[3] * qiw);
       const real qiub = sgn * g[3] * txz * h, qivb = sgn * g[3] * tyz * h, qiwb = sgn * g[3] * tzz * h;
@@ -1031,11 +1028,13 @@ __device__ __forceinline__ void diffusion_cell_adjoint(const real* __restrict__ 
     double gmu = 0.0;
     auto side = [&](auto blk, int sd, const real gw[4]) {
       constexpr int B = decltype(blk)::value;
+      // one pass gives the side's flux (for dS/dmu) and its transpose
       real out[4] = {0, 0, 0, 0}, gm4[4];
-      visc_side<real, false, B>(Sget, sd, oodx, oody, oodz, gw, out, Sacc);
 #pragma unroll
-      for (int c = 0; c < 4; ++c) { gmu += (double)gw[c] * (double)out[c]; gm4[c] = gw[c] * mu; }
+      for (int c = 0; c < 4; ++c) gm4[c] = gw[c] * mu;
       visc_side<real, true, B>(Sget, sd, oodx, oody, oodz, gm4, out, Sacc);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) gmu += (double)gw[c] * (double)out[c];
     };
     using BX = std::integral_constant<int, 0>;
     using BY = std::integral_constant<int, 1>;
