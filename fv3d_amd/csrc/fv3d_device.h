@@ -69,6 +69,24 @@ __device__ __forceinline__ float qdiv<float>(float a, float b) {
   return a * __builtin_amdgcn_rcpf(b);
 }
 
+// FP64 reciprocal and square root to about an ulp (not correctly rounded),
+// for FP64 arithmetic on FP32 data where the last bit does not matter
+__device__ __forceinline__ double recip_f64(double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  r = __builtin_fma(r, __builtin_fma(-b, r, 1.0), r);
+  return __builtin_fma(r, __builtin_fma(-b, r, 1.0), r);
+}
+__device__ __forceinline__ double sqrt_f64(double x) {
+  const double g = __builtin_amdgcn_rsq(x);
+  double s = x * g, h = 0.5 * g;
+  const double e = __builtin_fma(-h, s, 0.5);
+  s = __builtin_fma(s, e, s);
+  h = __builtin_fma(h, e, h);
+  s = __builtin_fma(__builtin_fma(-s, s, x), h, s);
+  // rsq(0) is inf: keep sqrt(0) = 0 and sqrt(x < 0) = NaN
+  return x > 0.0 ? s : (x == 0.0 ? 0.0 : __builtin_nan(""));
+}
+
 // t / h for a cell width h (1/h = ih): the product when h is a power of two
 // (the same bits) and in FP32, else the quotient
 template <typename real>

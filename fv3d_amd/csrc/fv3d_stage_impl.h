@@ -532,11 +532,23 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
       }
 
       if (inv_next) {  // next step's inv_dt_hyp from the stored state (ComputeDt.h:32-35)
-        const Prim<double> q = cons_to_prim<double>(u[0], u[1], u[2], u[3], u[4], P.gm1);
-        const double cs = sound_speed(q.r, q.p, P.gamma);
-        const double a = cs + fabs(q.u), bb = cs + fabs(q.v), cc = cs + fabs(q.w);
-        const double ih = (P.pow2x ? a * P.idx : a / P.dx) + (P.pow2y ? bb * P.idy : bb / P.dy) +
-                          (P.pow2z ? cc * P.idz : cc / P.dz);
+        double ih;
+        if (sizeof(real) == 8) {
+          const Prim<double> q = cons_to_prim<double>(u[0], u[1], u[2], u[3], u[4], P.gm1);
+          const double cs = sound_speed(q.r, q.p, P.gamma);
+          const double a = cs + fabs(q.u), bb = cs + fabs(q.v), cc = cs + fabs(q.w);
+          ih = (P.pow2x ? a * P.idx : a / P.dx) + (P.pow2y ? bb * P.idy : bb / P.dy) +
+               (P.pow2z ? cc * P.idz : cc / P.dz);
+        } else {
+          // FP32 state: the same FP64 expression, with its quotients and square
+          // root to about an ulp of FP64 (Newton-refined rcp and rsq) instead of
+          // the correctly rounded sequences; far below the state's FP32 rounding
+          const double r0 = u[0], ir = recip_f64(r0);
+          const double vx = u[1] * ir, vy = u[2] * ir, vz = u[3] * ir;
+          const double pr = ((double)u[4] - 0.5 * r0 * (vx * vx + vy * vy + vz * vz)) * P.gm1;
+          const double cs = sqrt_f64(pr * P.gamma * ir);
+          ih = (cs + fabs(vx)) * P.idx + (cs + fabs(vy)) * P.idy + (cs + fabs(vz)) * P.idz;
+        }
         if (ih > inv_max) inv_max = ih;
         if (!(ih <= 1.0e300)) bad = 1;  // NaN/inf: the reference's Max reducer drops NaN
       }
