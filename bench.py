@@ -244,7 +244,7 @@ def stratified_form(F, steps=2000):
     return out
 
 
-def adjoint_form(F, n=128, steps=200, K=20, ndirs=16, precision=None, fd=True):
+def adjoint_form(F, n=128, steps=200, K=20, ndirs=16, precision=None, fd=True, ref=None):
     """The discrete adjoint workload: n^3 periodic box, gamma 5/3, PLM +
     HLLC, SSP-RK2, CFL 0.4, conduction kappa=0.02 and viscosity mu=0.01;
     rho = 1 + 0.05 GRF (P(k) ~ k^-2), p = 1, solenoidal velocity of rms 0.1.
@@ -255,7 +255,9 @@ def adjoint_form(F, n=128, steps=200, K=20, ndirs=16, precision=None, fd=True):
     (eps = 1e-6) of J along `ndirs` unit-norm random directions of the
     initial density field (Gaussian random fields with the density's own
     k^-2 spectrum) against the adjoint's dJ/drho0, plus dJ/dkappa and dJ/dmu
-    against their own central differences."""
+    against their own central differences.  With `ref` (the FP64 call's
+    returned `_ref`), an FP32 call reuses its dt sequence and target and
+    reports its gradients' errors against the FP64 adjoint's."""
     from fv3d_amd import problems
     precision = F.FP64 if precision is None else precision
     p = F.Params.defaults().replace(
@@ -265,15 +267,18 @@ def adjoint_form(F, n=128, steps=200, K=20, ndirs=16, precision=None, fd=True):
         mu=0.01)
     Q0 = problems.turbulent_box(p, seed=12345)
     sim = F.Simulation(p, precision)
-    sim.set_prim(Q0)
-    dts = []
-    for _ in range(steps):
-        dts.append(sim.compute_dt()[0])
-        sim.step()
-    dts = np.array(dts)
-    sim.set_prim(problems.turbulent_box(p, seed=12346))
-    sim.forward_frozen(dts)
-    target = sim.get_prim(with_ghosts=False)[..., 0].copy()
+    if ref is None:
+        sim.set_prim(Q0)
+        dts = []
+        for _ in range(steps):
+            dts.append(sim.compute_dt()[0])
+            sim.step()
+        dts = np.array(dts)
+        sim.set_prim(problems.turbulent_box(p, seed=12346))
+        sim.forward_frozen(dts)
+        target = sim.get_prim(with_ghosts=False)[..., 0].copy()
+    else:
+        dts, target = ref["dts"], ref["target"]
     d0 = Q0[p.kbeg:p.kend, p.jbeg:p.jend, p.ibeg:p.iend]
     vel = d0[..., 1:4]
 
@@ -319,6 +324,11 @@ def adjoint_form(F, n=128, steps=200, K=20, ndirs=16, precision=None, fd=True):
                                "max_rel_diff_vs_recomputed": stored_rel},
            "J": J0,
            "dJ_dkappa": dk, "dJ_dmu": dm}
+    if ref is not None:
+        out["vs_fp64_adjoint"] = {
+            "dJ_drho0_rel_l2": float(np.linalg.norm(g - ref["g"]) / np.linalg.norm(ref["g"])),
+            "dJ_dkappa_rel": abs(dk - ref["dk"]) / abs(ref["dk"]),
+            "dJ_dmu_rel": abs(dm - ref["dm"]) / abs(ref["dm"])}
     if fd and ndirs > 0:
         rng = np.random.default_rng(7)
         eps = 1e-6
@@ -341,6 +351,7 @@ def adjoint_form(F, n=128, steps=200, K=20, ndirs=16, precision=None, fd=True):
             Jm = J_of(d0[..., 0], p.replace(**{key: val - h}))[0]
             out[f"fd_{key}_rel_err"] = abs((Jp - Jm) / (2 * h) - ad) / abs(ad)
     sim.close()
+    out["_ref"] = {"dts": dts, "target": target, "g": g, "dk": dk, "dm": dm}
     return out
 
 
@@ -430,8 +441,13 @@ def main():
     if args.stratified > 0:
         out.setdefault("forms", {})["stratified_tc_visc"] = stratified_form(F, args.stratified)
     if args.adjoint > 0:
-        out.setdefault("forms", {})["adjoint_fp64"] = adjoint_form(F, args.adjoint, args.adjoint_steps,
-                                                                   ndirs=args.adjoint_dirs)
+        a64 = adjoint_form(F, args.adjoint, args.adjoint_steps, ndirs=args.adjoint_dirs)
+        a32 = adjoint_form(F, args.adjoint, args.adjoint_steps, precision=F.FP32_PLAIN, fd=False,
+                           ref=a64["_ref"])
+        a64.pop("_ref")
+        a32.pop("_ref")
+        out.setdefault("forms", {})["adjoint_fp64"] = a64
+        out["forms"]["adjoint_fp32"] = a32
     if args.cpu_budget > 0:
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_budget)
     else:

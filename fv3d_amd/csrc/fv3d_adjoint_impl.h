@@ -693,7 +693,9 @@ k_adj_faces_tiled(const real* __restrict__ U, const real* __restrict__ v, real s
                   double dt, DevParams P, int color, int ntiles) {
   using T = FaceTile<real, DIR>;
   __shared__ real sq[5][T::NWIN];
-  __shared__ real sa[5][T::NWIN];
+  // adjoint accumulator in FP64 for both precisions: FP32 LDS atomic adds
+  // (ds_add_f32) measured 4 to 9 times slower than ds_add_f64 in these kernels
+  __shared__ double sa[5][T::NWIN];
   const int tid = threadIdx.x;
   const int Nd = DIR == FV3D_IX ? P.Nx : (DIR == FV3D_IY ? P.Ny : P.Nz);
   const int bc = DIR == FV3D_IX ? P.bcx : (DIR == FV3D_IY ? P.bcy : P.bcz);
@@ -809,7 +811,7 @@ k_adj_faces_tiled(const real* __restrict__ U, const real* __restrict__ v, real s
 #pragma unroll
       for (int r = 0; r < 4; ++r)
 #pragma unroll
-        for (int n = 0; n < 5; ++n) atomicAdd(&sa[n][wc[r]], qb[r][n]);
+        for (int n = 0; n < 5; ++n) atomicAdd(&sa[n][wc[r]], (double)qb[r][n]);
   }
   if (d0 == 0) {
     real qb[4][5];
@@ -820,7 +822,7 @@ k_adj_faces_tiled(const real* __restrict__ U, const real* __restrict__ v, real s
 #pragma unroll
       for (int r = 0; r < 4; ++r)
 #pragma unroll
-        for (int n = 0; n < 5; ++n) atomicAdd(&sa[n][wc[r]], qb[r][n]);
+        for (int n = 0; n < 5; ++n) atomicAdd(&sa[n][wc[r]], (double)qb[r][n]);
   }
   __syncthreads();
 
@@ -836,7 +838,7 @@ k_adj_faces_tiled(const real* __restrict__ U, const real* __restrict__ v, real s
       const long long src = base + (long long)map_coord(g, Nd, bc, fl) * stride;
 #pragma unroll
       for (int n = 0; n < 5; ++n) {
-        real x = sa[n][w];
+        real x = (real)sa[n][w];
         if (fl && n == 1 + DIR) x = -x;
         if (ghost || color < 0) atomic_add(&Qb[src + n * P.sf], x);
         else if (x != (real)0) Qb[src + n * P.sf] += x;
@@ -1062,7 +1064,7 @@ k_adj_diffusion_tiled(const real* __restrict__ U, const real* __restrict__ v, re
                       real* Qb, double dt, DevParams P, double* grad, int3 color, int3 ntiles) {
   constexpr int WX = AD_X + 2, WY = AD_Y + 2, WZ = AD_Z + 2, NW = WX * WY * WZ;
   __shared__ real sq[4][NW];  // u, v, w, T of the window (ghost-mapped)
-  __shared__ real sa[4][NW];  // their adjoints
+  __shared__ double sa[4][NW];  // their adjoints (FP64 in both precisions)
   const int tid = threadIdx.x;
   // the grid covers the tiles of one colour
   const int i0 = tile_of(blockIdx.x, color.x, ntiles.x) * AD_X;
@@ -1104,7 +1106,7 @@ k_adj_diffusion_tiled(const real* __restrict__ U, const real* __restrict__ v, re
     diffusion_cell_adjoint<real, TC, VISC>(
         v, sigma, dt, P, i, j, k, cell,
         [&](int dk, int dj, int di, int c) -> real { return sq[c][W(dk, dj, di)]; },
-        [&](int dk, int dj, int di, int c, real x) { atomicAdd(&sa[c][W(dk, dj, di)], x); }, gk, gm);
+        [&](int dk, int dj, int di, int c, real x) { atomicAdd(&sa[c][W(dk, dj, di)], (double)x); }, gk, gm);
   }
   __syncthreads();
   for (int ghosts = 0; ghosts < 2; ++ghosts) {
@@ -1125,11 +1127,11 @@ k_adj_diffusion_tiled(const real* __restrict__ U, const real* __restrict__ v, re
         else if (x != (real)0) Qb[a] += x;
       };
       if (VISC) {
-        put(src + 1 * P.sf, fx ? -sa[0][w] : sa[0][w]);
-        put(src + 2 * P.sf, fy ? -sa[1][w] : sa[1][w]);
-        put(src + 3 * P.sf, fz ? -sa[2][w] : sa[2][w]);
+        put(src + 1 * P.sf, (real)(fx ? -sa[0][w] : sa[0][w]));
+        put(src + 2 * P.sf, (real)(fy ? -sa[1][w] : sa[1][w]));
+        put(src + 3 * P.sf, (real)(fz ? -sa[2][w] : sa[2][w]));
       }
-      if (TC) put(src + 5 * P.sf, sa[3][w]);
+      if (TC) put(src + 5 * P.sf, (real)sa[3][w]);
     }
   }
   // parameter gradients: one partial per tile (summed by k_adj_reduce; a
@@ -1164,7 +1166,7 @@ k_adj_diffusion_march(const real* __restrict__ U, const real* __restrict__ v, re
                       real* Qb, double dt, DevParams P, double* grad, int nzc) {
   constexpr int WX = AZ_X + 2, WY = AZ_Y + 2, NWP = WX * WY;
   __shared__ real sq[3][4][NWP];  // u, v, w, T of three window planes (ghost-mapped)
-  __shared__ real sa[3][4][NWP];  // their adjoints
+  __shared__ double sa[3][4][NWP];  // their adjoints (FP64 in both precisions: ds_add_f64)
   const int tid = threadIdx.x;
   const int i0 = blockIdx.x * AZ_X, j0 = blockIdx.y * AZ_Y, k0 = blockIdx.z * nzc;
   const int k1 = min(k0 + nzc, P.Nz);
@@ -1200,11 +1202,11 @@ k_adj_diffusion_march(const real* __restrict__ U, const real* __restrict__ v, re
       const int si = map_coord(wi, P.Nx, P.bcx, fx), sj = map_coord(wj, P.Ny, P.bcy, fy);
       const long long src = si + (long long)sj * P.sy + (long long)sk * P.sz;
       if (VISC) {
-        atomic_add(&Qb[src + 1 * P.sf], fx ? -sa[sl][0][w] : sa[sl][0][w]);
-        atomic_add(&Qb[src + 2 * P.sf], fy ? -sa[sl][1][w] : sa[sl][1][w]);
-        atomic_add(&Qb[src + 3 * P.sf], fz ? -sa[sl][2][w] : sa[sl][2][w]);
+        atomic_add(&Qb[src + 1 * P.sf], (real)(fx ? -sa[sl][0][w] : sa[sl][0][w]));
+        atomic_add(&Qb[src + 2 * P.sf], (real)(fy ? -sa[sl][1][w] : sa[sl][1][w]));
+        atomic_add(&Qb[src + 3 * P.sf], (real)(fz ? -sa[sl][2][w] : sa[sl][2][w]));
       }
-      if (TC) atomic_add(&Qb[src + 5 * P.sf], sa[sl][3][w]);
+      if (TC) atomic_add(&Qb[src + 5 * P.sf], (real)sa[sl][3][w]);
     }
   };
 
@@ -1224,7 +1226,7 @@ k_adj_diffusion_march(const real* __restrict__ U, const real* __restrict__ v, re
       diffusion_cell_adjoint<real, TC, VISC>(
           v, sigma, dt, P, i, j, k, cell,
           [&](int dk, int dj, int di, int c) -> real { return sq[sl[dk]][c][W(dj, di)]; },
-          [&](int dk, int dj, int di, int c, real x) { atomicAdd(&sa[sl[dk]][c][W(dj, di)], x); },
+          [&](int dk, int dj, int di, int c, real x) { atomicAdd(&sa[sl[dk]][c][W(dj, di)], (double)x); },
           gkc, gmc);
       gk += gkc;
       gm += gmc;
