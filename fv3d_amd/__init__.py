@@ -14,6 +14,7 @@ from __future__ import annotations
 import ctypes
 import os
 import sys
+import warnings
 from dataclasses import dataclass
 from typing import Optional
 
@@ -35,6 +36,11 @@ VSC_CONSTANT = 0
 FP64, FP32, FP32_PLAIN = 0, 1, 2
 
 ERRORS = {1: "EINVAL", 2: "EHIP", 3: "ENOMEM", 4: "ENONFINITE"}
+ENONFINITE = 4
+
+
+class NonFiniteWarning(RuntimeWarning):
+    """The state holds a cell with a non-finite inverse time step."""
 
 
 class Params(ctypes.Structure):
@@ -224,6 +230,7 @@ class Simulation:
 
     def __init__(self, params: Params, precision: int = FP64, device: int = 0):
         self.params = params.copy()
+        self.nonfinite = False
         self._ctx = ctypes.c_void_p()
         _check(lib().fv3d_create(ctypes.byref(self.params), precision, device, ctypes.byref(self._ctx)))
 
@@ -252,12 +259,14 @@ class Simulation:
         if Q.shape != self._shape(with_ghosts):
             raise ValueError(f"Q has shape {Q.shape}, expected {self._shape(with_ghosts)}")
         _check(lib().fv3d_set_prim(self._ctx, _dptr(Q), int(with_ghosts)), self._ctx)
+        self.nonfinite = False
 
     def set_cons(self, U: np.ndarray, with_ghosts: bool = True):
         U = np.ascontiguousarray(U, dtype=np.float64)
         if U.shape != self._shape(with_ghosts):
             raise ValueError(f"U has shape {U.shape}, expected {self._shape(with_ghosts)}")
         _check(lib().fv3d_set_cons(self._ctx, _dptr(U), int(with_ghosts)), self._ctx)
+        self.nonfinite = False
 
     def get_prim(self, with_ghosts: bool = True) -> np.ndarray:
         out = np.zeros(self._shape(with_ghosts))
@@ -274,11 +283,25 @@ class Simulation:
         _check(lib().fv3d_get_cons(self._ctx, _dptr(out), int(with_ghosts)), self._ctx)
         return out
 
-    def compute_dt(self):
-        """Returns (dt, (inv_dt_hyp, inv_dt_par_tc, inv_dt_par_visc))."""
+    def compute_dt(self, strict: bool = False):
+        """Returns (dt, (inv_dt_hyp, inv_dt_par_tc, inv_dt_par_visc)).
+
+        A cell whose inverse time step is not finite (negative pressure or
+        zero density) is left out of the maximum, as the reference's Max
+        reducer leaves out NaN (ComputeDt.h:51-58), and its run goes on.  The
+        session then sets ``nonfinite`` and warns once; with ``strict`` it
+        raises FV3DError (ENONFINITE) instead."""
         inv = np.zeros(3)
         dt = ctypes.c_double()
-        _check(lib().fv3d_compute_dt(self._ctx, _dptr(inv), ctypes.byref(dt)), self._ctx)
+        rc = lib().fv3d_compute_dt(self._ctx, _dptr(inv), ctypes.byref(dt))
+        if rc == ENONFINITE and not strict:
+            if not self.nonfinite:
+                warnings.warn("fv3d: a cell has a non-finite inverse time step (negative pressure or "
+                              "zero density); it is left out of dt, as in the reference",
+                              NonFiniteWarning, stacklevel=2)
+            self.nonfinite = True
+        else:
+            _check(rc, self._ctx)
         return dt.value, tuple(inv)
 
     def step(self, n: int = 1):

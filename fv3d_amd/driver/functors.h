@@ -54,7 +54,16 @@ class ComputeDtFunctor {
   explicit ComputeDtFunctor(const fv3d_params& p) : params_(p) {}
   double computeDt(Device& d, double /*max_dt*/, double t, bool diag) const {
     double inv[3], dt;
-    check(fv3d_compute_dt(d.ctx(), inv, &dt), d.ctx());
+    const int rc = fv3d_compute_dt(d.ctx(), inv, &dt);
+    if (rc == FV3D_ENONFINITE) {
+      // the reference leaves such cells out of its Max reduction and goes on
+      if (!warned_)
+        std::fprintf(stderr, "fv3d: warning at t=%g: %s; left out of dt, as in the reference\n", t,
+                     fv3d_last_error(d.ctx()));
+      warned_ = true;
+    } else {
+      check(rc, d.ctx());
+    }
     if (diag)
       std::printf("Computing dts at (t=%g) : dt_hyp=%g; dt_TC=%g; dt_visc=%g\n", t, 1.0 / inv[0],
                   1.0 / inv[1], 1.0 / inv[2]);
@@ -63,6 +72,7 @@ class ComputeDtFunctor {
 
  private:
   fv3d_params params_;
+  mutable bool warned_ = false;
 };
 
 // Update.h:36-207: one Euler or SSP-RK2 step with the dt of the last

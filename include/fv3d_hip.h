@@ -69,7 +69,10 @@ int fv3d_get_cons(fv3d_ctx* ctx, double* U, int with_ghosts);
 /* ComputeDtFunctor::computeDt (ComputeDt.h:17-63) on prim(U) of the current
  * state: inv[3] = {inv_dt_hyp, inv_dt_par_tc, inv_dt_par_visc}, *dt =
  * CFL/max(inv).  Synchronises; returns FV3D_ENONFINITE if a cell produced a
- * non-finite inverse time step (the reference silently drops such cells). */
+ * non-finite inverse time step (negative pressure or zero density).  inv and
+ * *dt are written in that case too, from the finite cells: the reference's
+ * Max reducer drops NaN and its run goes on, so a caller that follows the
+ * reference treats this code as a warning and keeps stepping. */
 int fv3d_compute_dt(fv3d_ctx* ctx, double inv[3], double* dt);
 
 /* UpdateFunctor::update (Update.h:180-206) with dt = computeDt(prim(U)):
