@@ -981,6 +981,25 @@ __device__ __forceinline__ void diffusion_cell_adjoint(const real* __restrict__ 
                                                        int j, int k, long long cell, GET&& S,
                                                        ADD&& A, double& gk, double& gm) {
   const real rdt = (real)dt;
+  // The conduction terms and those of the four viscous sides every cell has
+  // are summed per (offset, component) in registers first: their 119 terms
+  // land on 51 distinct values, each added to the window once.  Entries no
+  // term reaches fold to the constant 0 and are skipped.
+  real acc[3][3][3][4];
+#pragma unroll
+  for (int e = 0; e < 108; ++e) (&acc[0][0][0][0])[e] = (real)0;
+  auto flush_acc = [&](auto& a, int nc) {
+#pragma unroll
+    for (int dk = 0; dk < 3; ++dk)
+#pragma unroll
+      for (int dj = 0; dj < 3; ++dj)
+#pragma unroll
+        for (int di = 0; di < 3; ++di)
+#pragma unroll
+          for (int c = 0; c < 4; ++c)
+            if (c < nc && !(__builtin_constant_p(a[dk][dj][di][c]) && a[dk][dj][di][c] == (real)0))
+              A(dk, dj, di, c, a[dk][dj][di][c]);
+  };
   if (TC) {
     const real Eb = sigma * v[cell + 4 * P.sf];
     // stencil offsets (dk, dj, di): centre, x-, x+, y-, y+, z-, z+
@@ -1014,7 +1033,7 @@ __device__ __forceinline__ void diffusion_cell_adjoint(const real* __restrict__ 
     gk = (double)Eb * (double)dE1;
     const real kf = (real)P.kface;
 #pragma unroll
-    for (int o = 0; o < 7; ++o) A(ok[o], oj[o], oi[o], 3, Eb * kf * c[o]);
+    for (int o = 0; o < 7; ++o) acc[ok[o]][oj[o]][oi[o]][3] += Eb * kf * c[o];
   }
   if (VISC) {
     // per side: x and y faces once (side 2 of the lower cell, carrying
@@ -1026,35 +1045,26 @@ __device__ __forceinline__ void diffusion_cell_adjoint(const real* __restrict__ 
 #pragma unroll
     for (int c = 0; c < 4; ++c) gi[c] = sdt * v[cell + (c + 1) * P.sf];
     auto Sget = [&](int dk, int dj, int di, int c) -> real { return S(dk, dj, di, c); };
-    auto Sacc = [&](int dk, int dj, int di, int c, real x) { A(dk, dj, di, c, x); };
     double gmu = 0.0;
-    auto side = [&](auto blk, int sd, const real gw[4]) {
+    // one pass gives the side's flux (for dS/dmu) and its transpose, summed into a
+    auto side = [&](auto blk, int sd, const real gw[4], auto& a) {
       constexpr int B = decltype(blk)::value;
-      // one pass gives the side's flux (for dS/dmu) and its transpose
       real out[4] = {0, 0, 0, 0}, gm4[4];
 #pragma unroll
       for (int c = 0; c < 4; ++c) gm4[c] = gw[c] * mu;
-      // the side's contributions are summed per (offset, component) in
-      // registers first (28 terms land on 22 distinct values), then added
-      // to the window once each
-      real acc[3][3][3][3];
-#pragma unroll
-      for (int e = 0; e < 81; ++e) (&acc[0][0][0][0])[e] = (real)0;
-      auto Sloc = [&](int dk, int dj, int di, int c, real x) { acc[dk][dj][di][c] += x; };
+      auto Sloc = [&](int dk, int dj, int di, int c, real x) { a[dk][dj][di][c] += x; };
       visc_side<real, true, B>(Sget, sd, oodx, oody, oodz, gm4, out, Sloc);
 #pragma unroll
       for (int c = 0; c < 4; ++c) gmu += (double)gw[c] * (double)out[c];
+    };
+    // the low-edge sides are summed and flushed on their own, so that the
+    // cell-wide sum keeps its compile-time zeros
+    auto edge_side = [&](auto blk, const real gw[4]) {
+      real e[3][3][3][4];
 #pragma unroll
-      for (int dk = 0; dk < 3; ++dk)
-#pragma unroll
-        for (int dj = 0; dj < 3; ++dj)
-#pragma unroll
-          for (int di = 0; di < 3; ++di)
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-              // entries no term reached fold to the constant 0 and are skipped
-              if (!(__builtin_constant_p(acc[dk][dj][di][c]) && acc[dk][dj][di][c] == (real)0))
-                Sacc(dk, dj, di, c, acc[dk][dj][di][c]);
+      for (int n = 0; n < 108; ++n) (&e[0][0][0][0])[n] = (real)0;
+      side(blk, 1, gw, e);
+      flush_acc(e, 3);
     };
     using BX = std::integral_constant<int, 0>;
     using BY = std::integral_constant<int, 1>;
@@ -1062,16 +1072,17 @@ __device__ __forceinline__ void diffusion_cell_adjoint(const real* __restrict__ 
     real gf[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) gf[c] = gi[c] - (i + 1 < P.Nx ? sdt * v[cell + 1 + (c + 1) * P.sf] : (real)0);
-    side(BX{}, 2, gf);
-    if (i == 0) side(BX{}, 1, gi);
+    side(BX{}, 2, gf, acc);
+    if (i == 0) edge_side(BX{}, gi);
 #pragma unroll
     for (int c = 0; c < 4; ++c) gf[c] = gi[c] - (j + 1 < P.Ny ? sdt * v[cell + P.sy + (c + 1) * P.sf] : (real)0);
-    side(BY{}, 2, gf);
-    if (j == 0) side(BY{}, 1, gi);
-    side(BZ{}, 1, gi);
-    side(BZ{}, 2, gi);
+    side(BY{}, 2, gf, acc);
+    if (j == 0) edge_side(BY{}, gi);
+    side(BZ{}, 1, gi, acc);
+    side(BZ{}, 2, gi, acc);
     gm = gmu;
   }
+  flush_acc(acc, 4);
 }
 
 constexpr int AD_X = 32, AD_Y = 4, AD_Z = 2;  // diffusion tile (256 threads: 3 workgroups per CU)
