@@ -1,7 +1,8 @@
 """The benchmarked workloads at their full sizes, through checks that do not
 depend on size: bit equality with the oracle over the first steps of the
-256^3 blast, exact mirror symmetry of the blast, and conservation of mass,
-momentum and energy to round-off on a periodic 256^3 box."""
+256^3 blast, exact mirror symmetry of the blast, conservation of mass,
+momentum and energy to round-off on a periodic 256^3 box, and a box whose
+state arrays pass 2^31 elements."""
 import numpy as np
 import pytest
 
@@ -80,3 +81,55 @@ def test_periodic_256_conservation(gpu_available, precision):
     for n, name in enumerate(["mass", "x-momentum", "y-momentum", "z-momentum", "energy"]):
         ref = max(abs(U0[n]), abs(U0[0]))
         assert abs(U1[n] - U0[n]) <= tol * ref, (name, U0[n], U1[n])
+
+
+def _pulse(shape, at, seed=7):
+    """Background rho = 1, p = 1 at rest, plus an 8^3 block of random
+    density, pressure and velocity whose low corner is `at` (z, y, x)."""
+    rng = np.random.default_rng(seed)
+    Q = np.empty(shape + (5,))
+    Q[...] = np.array([1.0, 0.0, 0.0, 0.0, 1.0])
+    b = Q[at[0]:at[0] + 8, at[1]:at[1] + 8, at[2]:at[2] + 8]
+    b[..., 0] = rng.uniform(0.5, 2.0, (8, 8, 8))
+    b[..., 1:4] = rng.normal(0.0, 0.5, (8, 8, 8, 3))
+    b[..., 4] = rng.uniform(0.5, 4.0, (8, 8, 8))
+    return Q
+
+
+@pytest.mark.parametrize("precision", [F.FP64, F.FP32])
+def test_state_past_32bit_offsets(gpu_available, precision):
+    """A 768 x 768 x 736 periodic box (434 M cells): each state array holds
+    2.17e9 values, so the last planes' offsets pass 2^31 elements (and
+    2^34 bytes in FP64).  A pulse placed in those planes, after two steps,
+    matches bit for bit the same pulse in a 32^3 box: the update of a cell
+    depends on its neighbourhood only, and the pulse sets dt in both."""
+    base = dict(gamma0=5.0 / 3.0, reconstruction=F.PLM, riemann_solver=F.HLLC, time_stepping=F.TS_RK2,
+                CFL=0.8, boundary_x=F.BC_PERIODIC, boundary_y=F.BC_PERIODIC, boundary_z=F.BC_PERIODIC,
+                tend=1.0)
+    # the same cell size, 1/32, in both boxes
+    small = F.Params.defaults().replace(Nx=32, Ny=32, Nz=32, **base)
+    big = F.Params.defaults().replace(Nx=768, Ny=768, Nz=736, xmax=24.0, ymax=24.0, zmax=23.0, **base)
+    assert (small.dx, small.dy, small.dz) == (big.dx, big.dy, big.dz)
+    assert 5 * big.Nx * big.Ny * big.Nz > 2 ** 31
+    at_s, at_b = (12, 12, 12), (big.Nz - 20, big.Ny - 20, big.Nx - 20)
+
+    def run(p, at):
+        Q0 = _pulse((p.Nz, p.Ny, p.Nx), at)
+        with F.Simulation(p, precision) as sim:
+            sim.set_prim(Q0, with_ghosts=False)
+            del Q0
+            dts = []
+            for _ in range(2):
+                dts.append(sim.compute_dt()[0])
+                sim.step()
+            Q = sim.get_prim(with_ghosts=False)
+        w = Q[at[0] - 10:at[0] + 18, at[1] - 10:at[1] + 18, at[2] - 10:at[2] + 18].copy()
+        return dts, w
+
+    print(f"32-bit offsets: small box, precision {precision}", flush=True)
+    dts_s, w_s = run(small, at_s)
+    print("32-bit offsets: 768x768x736 box", flush=True)
+    dts_b, w_b = run(big, at_b)
+    assert dts_s == dts_b
+    assert np.array_equal(w_s, w_b)
+    assert not np.array_equal(w_s[0], w_s[10])  # the pulse moved into the compared window
