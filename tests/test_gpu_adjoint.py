@@ -41,6 +41,11 @@ CONFIGS = {
     "pcm_hllc_euler_visc": dict(reconstruction=F.PCM, time_stepping=F.TS_EULER,
                                 thermal_conductivity_active=0, CFL=0.3),
     "periodic_hllc_plm_rk2_hydro": dict(thermal_conductivity_active=0, viscosity_active=0),
+    # the smallest extents the library takes (2 cells) along y, then along x
+    "thin_y_periodic_tc_visc": dict(Nx=20, Ny=2, Nz=3),
+    "thin_x_walls_hll_tc_visc": dict(Nx=2, Ny=17, Nz=5, riemann_solver=F.HLL,
+                                     boundary_x=F.BC_REFLECTING, boundary_y=F.BC_ABSORBING,
+                                     boundary_z=F.BC_REFLECTING),
 }
 
 

@@ -146,6 +146,18 @@ def test_periodic_tc_visc(gpu_available, recon, solver, ts):
     _compare_runs(p, smooth_random(p, 1), max_steps=6)
 
 
+@pytest.mark.parametrize("shape", [(24, 2, 3), (2, 3, 24), (3, 24, 2)])
+@pytest.mark.parametrize("bc", [F.BC_PERIODIC, F.BC_REFLECTING, F.BC_ABSORBING])
+def test_smallest_extents(gpu_available, shape, bc):
+    """Extents of 2 and 3 cells (the library takes >= 2): ghosts wrap or
+    mirror a 2-deep shell onto 2 cells, and tiles are mostly overhang."""
+    p = Params.defaults().replace(Nx=shape[0], Ny=shape[1], Nz=shape[2], boundary_x=bc, boundary_y=bc,
+                                  boundary_z=bc, reconstruction=F.PLM, riemann_solver=F.HLLC,
+                                  time_stepping=F.TS_RK2, thermal_conductivity_active=1, kappa=0.02,
+                                  viscosity_active=1, mu=0.01, tend=10.0)
+    _compare_runs(p, smooth_random(p, 5), max_steps=5)
+
+
 def test_hll_fixed_temperature_visc_gravity(gpu_available):
     """Partial tiles (16 < 64 columns), HLL, conduction with fixed-temperature
     walls, viscosity and gravity: the configuration family of the
