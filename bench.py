@@ -423,7 +423,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "kernel": "fv3d_hip::stage_kernel<double,8,PLM,HLLC,false,false>",
+            "kernel": "fv3d_hip::stage_kernel<double,8,PLM,HLLC,false,false,STAGE_PLAIN,STAGE_VAR_LEAN>",
             "alg_bytes_per_launch": bytes_per_launch,
             "mean_launch_ms": round(mean_launch_ms, 5),
         },
