@@ -121,14 +121,14 @@ def measure(F, p, Q0, precision, steps, warmup):
         if warmup > 0:
             sim.step(warmup)
         sim.synchronize()
-        dt0, _ = sim.compute_dt()
+        dt0, _ = sim.compute_dt(strict=True)
         if not np.isfinite(dt0) or dt0 <= 0:
             raise RuntimeError("non-finite dt after warm-up")
         t0 = time.perf_counter()
         ms, st = sim.time_steps(n, per_stage=True)
         sim.synchronize()
         wall += time.perf_counter() - t0
-        sim.compute_dt()  # raises if the window produced non-finite values
+        sim.compute_dt(strict=True)  # raises if the window produced non-finite values
         total_ms += ms
         stage_ms.extend(st.tolist())
         left -= n
@@ -225,7 +225,7 @@ def stratified_form(F, steps=2000):
             trip = np.zeros((steps, 3))
             t0 = time.perf_counter()
             for s_ in range(steps):
-                dt, inv = sim.compute_dt()
+                dt, inv = sim.compute_dt(strict=True)
                 trip[s_] = [1.0 / inv[0], 1.0 / inv[1], 1.0 / inv[2]]
                 sim.step()
             sim.synchronize()
@@ -271,7 +271,7 @@ def adjoint_form(F, n=128, steps=200, K=20, ndirs=16, precision=None, fd=True, r
         sim.set_prim(Q0)
         dts = []
         for _ in range(steps):
-            dts.append(sim.compute_dt()[0])
+            dts.append(sim.compute_dt(strict=True)[0])
             sim.step()
         dts = np.array(dts)
         sim.set_prim(problems.turbulent_box(p, seed=12346))

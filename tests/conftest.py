@@ -11,6 +11,8 @@ if ROOT not in sys.path:
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP kernels)")
     config.addinivalue_line("markers", "slow: long-running")
+    # a state that goes non-finite fails a test unless the test expects it (pytest.warns)
+    config.addinivalue_line("filterwarnings", "error::fv3d_amd.NonFiniteWarning")
 
 
 @pytest.fixture(scope="session")
