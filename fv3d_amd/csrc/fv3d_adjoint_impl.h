@@ -826,7 +826,9 @@ k_adj_faces_tiled(const real* __restrict__ U, const real* __restrict__ v, real s
   }
   __syncthreads();
 
-  for (int ghosts = 0; ghosts < 2; ++ghosts) {
+  // one pass of atomic flushes, or (colour passes) the plain updates of the
+  // tile's own cells and then the atomic ghost updates
+  for (int ghosts = 0; ghosts < (color < 0 ? 1 : 2); ++ghosts) {
     if (ghosts) __syncthreads();  // this block's own updates land first
     for (int w = tid; w < T::NWIN; w += blockDim.x) {
       int g; bool inside; long long base;
@@ -1138,7 +1140,7 @@ k_adj_diffusion_tiled(const real* __restrict__ U, const real* __restrict__ v, re
         [&](int dk, int dj, int di, int c, real x) { atomicAdd(&sa[c][W(dk, dj, di)], (double)x); }, gk, gm);
   }
   __syncthreads();
-  for (int ghosts = 0; ghosts < 2; ++ghosts) {
+  for (int ghosts = 0; ghosts < (color.x < 0 ? 1 : 2); ++ghosts) {
     if (ghosts) __syncthreads();  // this block's own updates land first
     for (int w = tid; w < NW; w += blockDim.x) {
       int wi, wj, wk;
