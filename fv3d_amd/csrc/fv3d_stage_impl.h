@@ -43,11 +43,12 @@ enum { STAGE_PLAIN = 0, STAGE_INC1 = 1, STAGE_INC2 = 2 };
 
 // Stage kernel variants.  STAGE_VAR_GENERAL tests the optional features at
 // run time.  STAGE_VAR_C91 adds the precomputed C91 z ghosts.  STAGE_VAR_LEAN
-// (PLM runs without gravity, PCM_WB, well-balanced or C91 z-flux overrides
-// or B02 conductivity, on cells whose sizes are powers of two) drops those
-// tests at compile time: the kernel holds fewer scalar values (about 30 fewer
-// SGPRs spilled to VGPR lanes in the FP64 blast kernel) and runs about 4%
-// faster, with the same bits (x/h == x*(1/h) when h is a power of two).
+// (PLM runs without PCM_WB, well-balanced or C91 z-flux overrides or B02
+// conductivity, on cells whose sizes are powers of two; gravity stays a
+// run-time switch) drops those tests at compile time: the kernel holds fewer
+// scalar values (about 30 fewer SGPRs spilled to VGPR lanes in the FP64
+// blast kernel) and runs about 4% faster, with the same bits (x/h ==
+// x*(1/h) when h is a power of two).
 enum { STAGE_VAR_GENERAL = 0, STAGE_VAR_C91 = 1, STAGE_VAR_LEAN = 2 };
 
 template <typename real>
@@ -142,9 +143,10 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
   constexpr bool LEAN = VAR == STAGE_VAR_LEAN;
   static_assert(!LEAN || RECON == FV3D_PLM, "the lean variant is PLM only");
   // optional features, constant false (or true for the power-of-two cell
-  // sizes) in the lean variant
+  // sizes) in the lean variant; gravity is tested at run time in every
+  // variant (keeping it out of the lean one measured no faster)
   const bool wb_zflux = !LEAN && P.wb_flux_zbc, c91_zflux = !LEAN && P.c91_flux_zbc;
-  const bool pcm_wb = !LEAN && P.pcm_wb, gravity = !LEAN && P.gravity;
+  const bool pcm_wb = !LEAN && P.pcm_wb, gravity = P.gravity;
   const bool p2x = LEAN || P.pow2x, p2y = LEAN || P.pow2y, p2z = LEAN || P.pow2z;
   const double* kztab = LEAN ? nullptr : P.kztab;
   // The primitive plane (and the temperature plane) are double-buffered by
@@ -641,7 +643,7 @@ static hipError_t launch_phys(const StageLaunch<real>& L, hipStream_t s) {
 
 // the lean variant applies (see STAGE_VAR_LEAN)
 inline bool stage_lean(const DevParams& P, int recon) {
-  return recon == FV3D_PLM && !P.pcm_wb && !P.wb_flux_zbc && !P.c91_flux_zbc && !P.gravity &&
+  return recon == FV3D_PLM && !P.pcm_wb && !P.wb_flux_zbc && !P.c91_flux_zbc &&
          P.pow2x && P.pow2y && P.pow2z && !P.kztab && P.bcz != FV3D_BC_C91;
 }
 
