@@ -333,6 +333,13 @@ static int enqueue_persistent(fv3d_batch* b) {
   L.riemann = p0.riemann_solver;
   L.tc = p0.thermal_conductivity_active != 0;
   L.visc = p0.viscosity_active != 0;
+  // the lean variant only if every box allows it, else the one any box allows
+  L.var = STAGE_VAR_LEAN;
+  for (const DevParams& d : b->dp) {
+    const int v = stage_variant(d, L.recon);
+    if (v == STAGE_VAR_GENERAL || v == STAGE_VAR_C91) { L.var = STAGE_VAR_GENERAL; break; }
+    if (v == STAGE_VAR_LEAN_ANYH) L.var = STAGE_VAR_LEAN_ANYH;
+  }
   const bool rk2 = p0.time_stepping == FV3D_TS_RK2;
   const bool k = bkahan(b);
   L.mode = k ? 1 : 0;

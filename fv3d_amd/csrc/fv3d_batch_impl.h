@@ -49,7 +49,7 @@ __device__ __forceinline__ void stage_pointers(const BatchBufs& bb, long long of
   }
 }
 
-template <typename real, int TY, int RECON, int RS, bool TC, bool VISC, int MODE>
+template <typename real, int TY, int RECON, int RS, bool TC, bool VISC, int MODE, int VAR>
 __global__ void __launch_bounds__(64 * (TY + 4))
 batch_stage_kernel(const BoxDesc* __restrict__ boxes, const int2* __restrict__ items,
                    const int* __restrict__ qstart, unsigned* counters,
@@ -90,34 +90,44 @@ batch_stage_kernel(const BoxDesc* __restrict__ boxes, const int2* __restrict__ i
     const unsigned long long* inv_cur = ring + (step % 3);
     unsigned long long* inv_next = last_stage ? ring + ((step + 1) % 3) : nullptr;
     unsigned long long* inv_zero = stage == 0 ? ring + ((step + 2) % 3) : nullptr;
-    stage_tile<real, TY, RECON, RS, TC, VISC, MODE>(Uin, Urk, Uout, Cio, inv_cur, inv_next,
-                                                    inv_zero, bb.health, P, wi.y);
+    stage_tile<real, TY, RECON, RS, TC, VISC, MODE, VAR>(Uin, Urk, Uout, Cio, inv_cur, inv_next,
+                                                         inv_zero, bb.health, P, wi.y);
   }
 }
 
-template <typename real, int TY, int RECON, int RS, bool TC, bool VISC, int MODE>
+template <typename real, int TY, int RECON, int RS, bool TC, bool VISC, int MODE, int VAR>
 static hipError_t batch_launch_t(const BatchLaunch& L, hipStream_t s) {
-  hipLaunchKernelGGL((batch_stage_kernel<real, TY, RECON, RS, TC, VISC, MODE>), dim3(L.grid),
+  hipLaunchKernelGGL((batch_stage_kernel<real, TY, RECON, RS, TC, VISC, MODE, VAR>), dim3(L.grid),
                      dim3(64 * (TY + 4)), 0, s, L.boxes, L.items, L.qstart, L.counters, L.step,
                      L.bufs, L.stage, L.ts, L.last_stage);
   return hipGetLastError();
 }
 
-template <typename real, int RECON, int RS, int MODE>
+template <typename real, int RECON, int RS, int MODE, int VAR>
 static hipError_t batch_launch_phys(const BatchLaunch& L, hipStream_t s) {
-  if (L.tc && L.visc) return batch_launch_t<real, STAGE_TY_VISC, RECON, RS, true, true, MODE>(L, s);
-  if (L.tc) return batch_launch_t<real, STAGE_TY, RECON, RS, true, false, MODE>(L, s);
-  if (L.visc) return batch_launch_t<real, STAGE_TY_VISC, RECON, RS, false, true, MODE>(L, s);
-  return batch_launch_t<real, STAGE_TY, RECON, RS, false, false, MODE>(L, s);
+  if (L.tc && L.visc) return batch_launch_t<real, STAGE_TY_VISC, RECON, RS, true, true, MODE, VAR>(L, s);
+  if (L.tc) return batch_launch_t<real, STAGE_TY, RECON, RS, true, false, MODE, VAR>(L, s);
+  if (L.visc) return batch_launch_t<real, STAGE_TY_VISC, RECON, RS, false, true, MODE, VAR>(L, s);
+  return batch_launch_t<real, STAGE_TY, RECON, RS, false, false, MODE, VAR>(L, s);
 }
 
+// L.var: the stage variant every box allows (the ensemble rejects C91)
 template <typename real, int MODE>
 static hipError_t batch_launch_mode(const BatchLaunch& L, hipStream_t s) {
   const bool plm = L.recon == FV3D_PLM, hll = L.riemann == FV3D_HLL;
-  if (plm && hll) return batch_launch_phys<real, FV3D_PLM, FV3D_HLL, MODE>(L, s);
-  if (plm) return batch_launch_phys<real, FV3D_PLM, FV3D_HLLC, MODE>(L, s);
-  if (hll) return batch_launch_phys<real, FV3D_PCM, FV3D_HLL, MODE>(L, s);
-  return batch_launch_phys<real, FV3D_PCM, FV3D_HLLC, MODE>(L, s);
+  if (L.var == STAGE_VAR_LEAN) {
+    if (hll) return batch_launch_phys<real, FV3D_PLM, FV3D_HLL, MODE, STAGE_VAR_LEAN>(L, s);
+    return batch_launch_phys<real, FV3D_PLM, FV3D_HLLC, MODE, STAGE_VAR_LEAN>(L, s);
+  }
+  if (L.var == STAGE_VAR_LEAN_ANYH) {
+    if (hll) return batch_launch_phys<real, FV3D_PLM, FV3D_HLL, MODE, STAGE_VAR_LEAN_ANYH>(L, s);
+    return batch_launch_phys<real, FV3D_PLM, FV3D_HLLC, MODE, STAGE_VAR_LEAN_ANYH>(L, s);
+  }
+  constexpr int G = STAGE_VAR_GENERAL;
+  if (plm && hll) return batch_launch_phys<real, FV3D_PLM, FV3D_HLL, MODE, G>(L, s);
+  if (plm) return batch_launch_phys<real, FV3D_PLM, FV3D_HLLC, MODE, G>(L, s);
+  if (hll) return batch_launch_phys<real, FV3D_PCM, FV3D_HLL, MODE, G>(L, s);
+  return batch_launch_phys<real, FV3D_PCM, FV3D_HLLC, MODE, G>(L, s);
 }
 
 }  // namespace fv3d_hip

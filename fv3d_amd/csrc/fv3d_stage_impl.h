@@ -41,16 +41,6 @@ constexpr int NC = WX + 4;  // window columns
 // kernel (the compensated state alternates between two arrays).
 enum { STAGE_PLAIN = 0, STAGE_INC1 = 1, STAGE_INC2 = 2 };
 
-// Stage kernel variants.  STAGE_VAR_GENERAL tests the optional features at
-// run time.  STAGE_VAR_C91 adds the precomputed C91 z ghosts.  STAGE_VAR_LEAN
-// (PLM runs without PCM_WB, well-balanced or C91 z-flux overrides or B02
-// conductivity, on cells whose sizes are powers of two; gravity stays a
-// run-time switch) drops those tests at compile time: the kernel holds fewer
-// scalar values (about 30 fewer SGPRs spilled to VGPR lanes in the FP64
-// blast kernel) and runs about 4% faster, with the same bits (x/h ==
-// x*(1/h) when h is a power of two).
-enum { STAGE_VAR_GENERAL = 0, STAGE_VAR_C91 = 1, STAGE_VAR_LEAN = 2 };
-
 template <typename real>
 __device__ __forceinline__ Prim<real> load_prim(const real* __restrict__ U, long long base,
                                                 long long sf, real gm1, bool fx, bool fy,
@@ -140,14 +130,15 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
                                            const DevParams& P, const int tile) {
   constexpr int NR = TY + 4;
   constexpr bool C91Z = VAR == STAGE_VAR_C91;
-  constexpr bool LEAN = VAR == STAGE_VAR_LEAN;
-  static_assert(!LEAN || RECON == FV3D_PLM, "the lean variant is PLM only");
+  constexpr bool LEAN = VAR == STAGE_VAR_LEAN || VAR == STAGE_VAR_LEAN_ANYH;
+  constexpr bool POW2 = VAR == STAGE_VAR_LEAN;
+  static_assert(!LEAN || RECON == FV3D_PLM, "the lean variants are PLM only");
   // optional features, constant false (or true for the power-of-two cell
   // sizes) in the lean variant; gravity is tested at run time in every
   // variant (keeping it out of the lean one measured no faster)
   const bool wb_zflux = !LEAN && P.wb_flux_zbc, c91_zflux = !LEAN && P.c91_flux_zbc;
   const bool pcm_wb = !LEAN && P.pcm_wb, gravity = P.gravity;
-  const bool p2x = LEAN || P.pow2x, p2y = LEAN || P.pow2y, p2z = LEAN || P.pow2z;
+  const bool p2x = POW2 || P.pow2x, p2y = POW2 || P.pow2y, p2z = POW2 || P.pow2z;
   const double* kztab = LEAN ? nullptr : P.kztab;
   // The primitive plane (and the temperature plane) are double-buffered by
   // plane parity unless viscosity is on, so the next plane can be published
@@ -641,12 +632,6 @@ static hipError_t launch_phys(const StageLaunch<real>& L, hipStream_t s) {
   return launch_t<real, STAGE_TY, RECON, RS, false, false, MODE, VAR>(L, s);
 }
 
-// the lean variant applies (see STAGE_VAR_LEAN)
-inline bool stage_lean(const DevParams& P, int recon) {
-  return recon == FV3D_PLM && !P.pcm_wb && !P.wb_flux_zbc && !P.c91_flux_zbc &&
-         P.pow2x && P.pow2y && P.pow2z && !P.kztab && P.bcz != FV3D_BC_C91;
-}
-
 // fillC91 (BoundaryConditions.h:86-134) for the z-ghost planes of every
 // domain column, from the stage's input state, as primitives.  The
 // reference multiplies by the gravity flag where g belongs (its Q17); so
@@ -707,16 +692,22 @@ static hipError_t launch_mode(const StageLaunch<real>& L, hipStream_t s) {
   }
   const bool plm = L.recon == FV3D_PLM;
   const bool hll = L.riemann == FV3D_HLL;
-  constexpr int G = STAGE_VAR_GENERAL, C = STAGE_VAR_C91, N = STAGE_VAR_LEAN;
-  if (L.P.bcz == FV3D_BC_C91) {
+  constexpr int G = STAGE_VAR_GENERAL, C = STAGE_VAR_C91, N = STAGE_VAR_LEAN,
+                H = STAGE_VAR_LEAN_ANYH;
+  const int var = stage_variant(L.P, L.recon);
+  if (var == C) {
     if (plm && hll) return launch_phys<real, FV3D_PLM, FV3D_HLL, MODE, C>(L, s);
     if (plm) return launch_phys<real, FV3D_PLM, FV3D_HLLC, MODE, C>(L, s);
     if (hll) return launch_phys<real, FV3D_PCM, FV3D_HLL, MODE, C>(L, s);
     return launch_phys<real, FV3D_PCM, FV3D_HLLC, MODE, C>(L, s);
   }
-  if (stage_lean(L.P, L.recon)) {
+  if (var == N) {
     if (hll) return launch_phys<real, FV3D_PLM, FV3D_HLL, MODE, N>(L, s);
     return launch_phys<real, FV3D_PLM, FV3D_HLLC, MODE, N>(L, s);
+  }
+  if (var == H) {
+    if (hll) return launch_phys<real, FV3D_PLM, FV3D_HLL, MODE, H>(L, s);
+    return launch_phys<real, FV3D_PLM, FV3D_HLLC, MODE, H>(L, s);
   }
   if (plm && hll) return launch_phys<real, FV3D_PLM, FV3D_HLL, MODE, G>(L, s);
   if (plm) return launch_phys<real, FV3D_PLM, FV3D_HLLC, MODE, G>(L, s);
