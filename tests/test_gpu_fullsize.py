@@ -83,6 +83,32 @@ def test_periodic_256_conservation(gpu_available, precision):
         assert abs(U1[n] - U0[n]) <= tol * ref, (name, U0[n], U1[n])
 
 
+@pytest.mark.parametrize("precision", [F.FP64, F.FP32])
+def test_stratified_full_size_conserves_mass(gpu_available, precision):
+    """The stratified-convection workload at its full size (256 x 256 x 128,
+    HLL, conduction with fixed-temperature walls, viscosity, gravity,
+    periodic x/y, reflecting z), 20 steps: gravity, conduction and the wall
+    terms change momentum and energy, but no term moves mass across the
+    reflecting walls, so the total mass stays to round-off (FP64 sums).
+    Momentum is not checked: the reference's viscous z branch (Viscosity.h
+    :122-133) does not evaluate a z face's stress alike from its two cells."""
+    p = F.Params.defaults().replace(
+        Nx=256, Ny=256, Nz=128, xmax=2.0, ymax=2.0, boundary_x=F.BC_PERIODIC, boundary_y=F.BC_PERIODIC,
+        boundary_z=F.BC_REFLECTING, reconstruction=F.PLM, riemann_solver=F.HLL, time_stepping=F.TS_RK2,
+        gamma0=5.0 / 3.0, gravity=1, g=1.0, CFL=0.4, thermal_conductivity_active=1, kappa=0.07,
+        bctc_zmin=F.BCTC_FIXED_TEMPERATURE, bctc_zmin_value=1.0, bctc_zmax=F.BCTC_FIXED_TEMPERATURE,
+        bctc_zmax_value=1.5, viscosity_active=1, mu=0.005, tend=100.0)
+    Q0 = F.problems.polytrope_convection(p)
+    with F.Simulation(p, precision) as sim:
+        sim.set_prim(Q0)
+        m0 = sim.get_cons(with_ghosts=False)[..., 0].sum()
+        sim.step(20)
+        U1 = sim.get_cons(with_ghosts=False)
+    assert np.isfinite(U1).all()
+    tol = 1e-12 if precision == F.FP64 else 2e-6
+    assert abs(U1[..., 0].sum() - m0) <= tol * m0, (m0, U1[..., 0].sum())
+
+
 def _pulse(shape, at, seed=7):
     """Background rho = 1, p = 1 at rest, plus an 8^3 block of random
     density, pressure and velocity whose low corner is `at` (z, y, x)."""
