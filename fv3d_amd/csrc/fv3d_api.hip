@@ -8,8 +8,8 @@
 //   Euler: stage(A -> B), swap(A, B)   reads slot s,   reduces s+1, clears s+2
 // The FP32 compensated RK2 adds a third state buffer C and a compensation
 // array:
-//          inc1(A -> K in B)           reads slot s,   clears slot s+2
-//          inc2(A, K, comp -> C), swap(A, C)   reduces slot s+1
+//          inc1(A -> U* in B)          reads slot s,   clears slot s+2
+//          inc2(B, rk=A, comp -> C), swap(A, C)   reduces slot s+1
 // so dt never round-trips through the host and consecutive steps need no
 // synchronisation.  The slot cleared by step s was last read by step s-1 and
 // is next written by step s+1, so clearing it inside step s races with
