@@ -136,18 +136,15 @@ static int ensure_qbuf(fv3d_ctx* c) {
 }
 
 // prim of a device state buffer into host memory
-static int state_prim_to_host(fv3d_ctx* c, const void* U, const void* K, double* Q,
-                              int with_ghosts) {
+static int state_prim_to_host(fv3d_ctx* c, const void* U, double* Q, int with_ghosts) {
   int rc = ensure_qbuf(c);
   if (rc) return rc;
   const fv3d_params& p = c->p;
   const size_t nt = (size_t)p.Ntx * p.Nty * p.Ntz;
   if (c->precision == FV3D_FP64)
-    HIPCHK(c, launch_cons_to_prim_ghosted<double>((const double*)U, (const double*)K, c->qbuf,
-                                                  c->dp, p.Ng, c->stream));
+    HIPCHK(c, launch_cons_to_prim_ghosted<double>((const double*)U, c->qbuf, c->dp, p.Ng, c->stream));
   else
-    HIPCHK(c, launch_cons_to_prim_ghosted<float>((const float*)U, (const float*)K, c->qbuf, c->dp,
-                                                 p.Ng, c->stream));
+    HIPCHK(c, launch_cons_to_prim_ghosted<float>((const float*)U, c->qbuf, c->dp, p.Ng, c->stream));
   if (with_ghosts) HIPCHK(c, launch_fill_boundaries(c->qbuf, p, c->stream));
   std::vector<double> h(5 * nt);
   HIPCHK(c, hipMemcpyAsync(h.data(), c->qbuf, 5 * nt * sizeof(double), hipMemcpyDeviceToHost,
@@ -241,14 +238,14 @@ int fv3d_get_prim(fv3d_ctx* c, double* Q, int with_ghosts) {
   if (!c || !Q) return fail(c, FV3D_EINVAL, "null argument");
   if (!c->have_state) return fail(c, FV3D_EINVAL, "no state set");
   HIPCHK(c, hipSetDevice(c->device));
-  return state_prim_to_host(c, c->A, nullptr, Q, with_ghosts);
+  return state_prim_to_host(c, c->A, Q, with_ghosts);
 }
 
 int fv3d_get_stage_prim(fv3d_ctx* c, double* Q, int with_ghosts) {
   if (!c || !Q) return fail(c, FV3D_EINVAL, "null argument");
   if (!c->have_stage) return fail(c, FV3D_EINVAL, "no step taken yet");
   HIPCHK(c, hipSetDevice(c->device));
-  return state_prim_to_host(c, c->B, nullptr, Q, with_ghosts);
+  return state_prim_to_host(c, c->B, Q, with_ghosts);
 }
 
 int fv3d_compute_dt(fv3d_ctx* c, double inv[3], double* dt) {

@@ -12,8 +12,8 @@ namespace fv3d_hip {
 // prim <-> cons
 // ---------------------------------------------------------------------------
 template <typename real>
-__global__ void k_cons_to_prim_ghosted(const real* __restrict__ U, const real* __restrict__ K,
-                                       double* __restrict__ Q, DevParams P, int Ng) {
+__global__ void k_cons_to_prim_ghosted(const real* __restrict__ U, double* __restrict__ Q, DevParams P,
+                                       int Ng) {
   const long long n = (long long)P.Nx * P.Ny * P.Nz;
   const long long Ntx = P.Nx + 2 * Ng, Nty = P.Ny + 2 * Ng, Ntz = P.Nz + 2 * Ng;
   const long long qf = Ntx * Nty * Ntz;
@@ -24,7 +24,7 @@ __global__ void k_cons_to_prim_ghosted(const real* __restrict__ U, const real* _
     const int k = (int)(c / ((long long)P.Nx * P.Ny));
     const long long s = i + j * P.sy + k * P.sz;
     real u[5];
-    for (int f = 0; f < 5; ++f) u[f] = K ? (real)(U[s + f * P.sf] + K[s + f * P.sf]) : U[s + f * P.sf];
+    for (int f = 0; f < 5; ++f) u[f] = U[s + f * P.sf];
     const Prim<double> q = cons_to_prim<double>(u[0], u[1], u[2], u[3], u[4], P.gm1);
     const long long d = (i + Ng) + (j + Ng) * Ntx + (k + Ng) * Ntx * Nty;
     Q[d] = q.r; Q[d + qf] = q.u; Q[d + 2 * qf] = q.v; Q[d + 3 * qf] = q.w; Q[d + 4 * qf] = q.p;
@@ -289,11 +289,11 @@ __global__ void k_zero_u64(unsigned long long* p, int n) {
 // launchers
 // ---------------------------------------------------------------------------
 template <typename real>
-hipError_t launch_cons_to_prim_ghosted(const real* U, const real* K, double* Q, const DevParams& P,
-                                       int Ng, hipStream_t s) {
+hipError_t launch_cons_to_prim_ghosted(const real* U, double* Q, const DevParams& P, int Ng,
+                                       hipStream_t s) {
   const long long n = (long long)P.Nx * P.Ny * P.Nz;
-  hipLaunchKernelGGL((k_cons_to_prim_ghosted<real>), dim3(nblocks(n, 256)), dim3(256), 0, s, U, K,
-                     Q, P, Ng);
+  hipLaunchKernelGGL((k_cons_to_prim_ghosted<real>), dim3(nblocks(n, 256)), dim3(256), 0, s, U, Q, P,
+                     Ng);
   return hipGetLastError();
 }
 
@@ -327,10 +327,10 @@ hipError_t launch_fill_zero_u64(unsigned long long* p, int n, hipStream_t s) {
   return hipGetLastError();
 }
 
-template hipError_t launch_cons_to_prim_ghosted<double>(const double*, const double*, double*,
-                                                        const DevParams&, int, hipStream_t);
-template hipError_t launch_cons_to_prim_ghosted<float>(const float*, const float*, double*,
-                                                       const DevParams&, int, hipStream_t);
+template hipError_t launch_cons_to_prim_ghosted<double>(const double*, double*, const DevParams&, int,
+                                                        hipStream_t);
+template hipError_t launch_cons_to_prim_ghosted<float>(const float*, double*, const DevParams&, int,
+                                                       hipStream_t);
 template hipError_t launch_prim_to_cons<double>(const double*, double*, const DevParams&,
                                                 hipStream_t);
 template hipError_t launch_prim_to_cons<float>(const double*, float*, const DevParams&,

@@ -275,9 +275,9 @@ int fv3d_batch_get_prim(fv3d_batch* b, int box, double* Q, int with_ghosts) {
   }
   void* U = box_state(b, box);
   if (b->precision == FV3D_FP64)
-    BCHK(b, launch_cons_to_prim_ghosted<double>((const double*)U, nullptr, b->qbuf, b->dp[box], p.Ng, b->stream));
+    BCHK(b, launch_cons_to_prim_ghosted<double>((const double*)U, b->qbuf, b->dp[box], p.Ng, b->stream));
   else
-    BCHK(b, launch_cons_to_prim_ghosted<float>((const float*)U, nullptr, b->qbuf, b->dp[box], p.Ng, b->stream));
+    BCHK(b, launch_cons_to_prim_ghosted<float>((const float*)U, b->qbuf, b->dp[box], p.Ng, b->stream));
   if (with_ghosts) BCHK(b, launch_fill_boundaries(b->qbuf, p, b->stream));
   std::vector<double> h(5 * nt);
   BCHK(b, hipMemcpyAsync(h.data(), b->qbuf, 5 * nt * sizeof(double), hipMemcpyDeviceToHost, b->stream));

@@ -61,7 +61,7 @@ struct BoxDesc {
 
 struct BatchBufs {
   void* A;     // state U^n (RK2 plain: updated in place by stage 2)
-  void* B;     // U* (plain RK2), K (compensated RK2), or the other Euler state
+  void* B;     // U* (RK2, plain or compensated), or the other Euler state
   void* C;     // compensated RK2: the other U buffer
   void* comp;  // compensated RK2: Kahan compensation
   int* health;
@@ -113,11 +113,11 @@ struct AdjLaunch {
 template <typename real>
 hipError_t launch_adjoint_stage(const AdjLaunch<real>& L, hipStream_t s);
 
-// cons -> prim over the domain of U (or of U + K when K is given), written
-// into a ghosted SoA array [5][Ntz][Nty][Ntx] (ghosts untouched)
+// cons -> prim over the domain of U, written into a ghosted SoA array
+// [5][Ntz][Nty][Ntx] (ghosts untouched)
 template <typename real>
-hipError_t launch_cons_to_prim_ghosted(const real* U, const real* K, double* Q, const DevParams& P,
-                                       int Ng, hipStream_t s);
+hipError_t launch_cons_to_prim_ghosted(const real* U, double* Q, const DevParams& P, int Ng,
+                                       hipStream_t s);
 // prim (SoA domain [5][Nz][Ny][Nx], double) -> cons state
 template <typename real>
 hipError_t launch_prim_to_cons(const double* Q, real* U, const DevParams& P, hipStream_t s);
