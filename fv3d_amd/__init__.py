@@ -414,6 +414,11 @@ class Batch:
         return out
 
     def compute_dt(self) -> np.ndarray:
+        """The next dt of every box.  Raises FV3DError (ENONFINITE) when a
+        box holds a cell with a non-finite inverse time step: unlike a single
+        run, which goes on as the reference does, an ensemble has no
+        counterpart to follow, and one broken box among hundreds would
+        otherwise pass unnoticed."""
         dts = np.zeros(len(self.params))
         self._check(lib().fv3d_batch_compute_dt(self._b, _dptr(dts)))
         return dts
