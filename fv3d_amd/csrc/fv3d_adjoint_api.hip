@@ -190,6 +190,7 @@ int adjoint_run_t(fv3d_ctx* c, int nsteps, int K, const double* dts_dev,
   A.simple = std::getenv("FV3D_ADJ_SIMPLE") != nullptr;
   A.colored = std::getenv("FV3D_ADJ_COLORED") != nullptr;
   A.colored_diffusion = A.colored;
+  A.no_fused_finalize = std::getenv("FV3D_ADJ_UNFUSED") != nullptr;
   for (int cp = ncp - 1; cp >= 0; --cp) {
     const int s0 = cp * K, s1 = std::min(nsteps, s0 + K);
     // the segment's states: S0[i] = U^{s0+i}, S1[i] = U1 of step s0+i

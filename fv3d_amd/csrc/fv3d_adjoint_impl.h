@@ -859,10 +859,50 @@ k_adj_faces_tiled(const real* __restrict__ U, const real* __restrict__ v, real s
 // -1 + chunk*F; they reach cells m0-1 .. m1+1.
 constexpr int AL_T = 4;  // lines (transverse coordinates) per 256-thread workgroup
 
-template <typename real, int DIR, int RECON, int RS>
+// The last step of an adjoint stage for one cell: out = sigma*v + J_c2p^T qb
+// + alpha*extra, with qb the cell's primitive-space adjoint (rho, u, v, w,
+// p, T) and q its primitives.
+template <typename real>
+__device__ __forceinline__ void finalize_cell(const real q[5], real qb[5], real Tb, const real vc[5],
+                                              real sigma, const real* __restrict__ extra, real alpha,
+                                              real* out, long long t, long long sf, real gm1) {
+  // T = p / rho
+  qb[4] += Tb / q[0];
+  qb[0] -= Tb * q[4] / (q[0] * q[0]);
+  // consToPrim (States.h:34-45) reversed
+  const real r = q[0], u = q[1], vv = q[2], w = q[3];
+  const real Ub1 = qb[1] / r - qb[4] * gm1 * u;
+  const real Ub2 = qb[2] / r - qb[4] * gm1 * vv;
+  const real Ub3 = qb[3] / r - qb[4] * gm1 * w;
+  const real Ub4 = qb[4] * gm1;
+  const real Ub0 = qb[0] - (qb[1] * u + qb[2] * vv + qb[3] * w) / r +
+                   qb[4] * gm1 * (real)0.5 * (u * u + vv * vv + w * w);
+  const real Ub[5] = {Ub0, Ub1, Ub2, Ub3, Ub4};
+#pragma unroll
+  for (int n = 0; n < 5; ++n) {
+    real o = sigma * vc[n] + Ub[n];
+    if (extra) o += alpha * extra[t + n * sf];
+    out[t + n * sf] = o;
+  }
+}
+
+// z-line chunks of F faces: cell k is reached by one chunk only, and by no
+// ghost image, when it lies 2 or more cells inside its chunk's face range
+// and 2 or more cells inside the domain.  The z-face kernel finalizes those
+// cells itself; k_adj_finalize does the rest.
+__device__ __forceinline__ bool z_exclusive(int k, int Nz, int F) {
+  if (k < 2 || k > Nz - 3) return false;
+  const int m0 = -1 + ((k + 1) / F) * F;
+  const int m1 = min(m0 + F, Nz);
+  return k >= m0 + 2 && k <= m1 - 2;
+}
+
+template <typename real, int DIR, int RECON, int RS, bool FIN = false>
 __global__ void __launch_bounds__(64 * AL_T)
 k_adj_faces_line(const real* __restrict__ U, const real* __restrict__ v, real sigma, real* Qb,
-                 double dt, DevParams P, int F) {
+                 double dt, DevParams P, int F, const real* __restrict__ extra = nullptr,
+                 real alpha = 0, real* out = nullptr) {
+  static_assert(!FIN || DIR == FV3D_IZ, "only the z faces, launched last, finalize");
   static_assert(DIR == FV3D_IY || DIR == FV3D_IZ, "x faces use the tiled kernel");
   constexpr bool ZORD = DIR == FV3D_IZ;
   const int Nd = DIR == FV3D_IY ? P.Ny : P.Nz;
@@ -897,6 +937,7 @@ k_adj_faces_line(const real* __restrict__ U, const real* __restrict__ v, real si
       atomic_add(&Qb[src[o] + n * P.sf], x);
     }
   };
+  real va[5] = {0, 0, 0, 0, 0};  // v of cell m-1 (FIN)
   load(0, m0 - 1);
   load(1, m0);
   load(2, m0 + 1);
@@ -949,7 +990,24 @@ k_adj_faces_line(const real* __restrict__ U, const real* __restrict__ v, real si
       slope_vjp(selb, ssb, acc[0], acc[1], acc[2]);
       slope_vjp(selc, ssc, acc[1], acc[2], acc[3]);
     }
-    flush(0);  // cell m-1: no later face of the chunk reaches it
+    // cell m-1: no later face of the chunk reaches it
+    if (FIN && z_exclusive(m - 1, Nd, F)) {
+      // every other term of its adjoint is in Qb already, and no other
+      // thread adds to it: finalize it here (same sums as a flush followed
+      // by k_adj_finalize) and clear its Qb
+      const long long c0 = src[0];
+      real qb[5];
+#pragma unroll
+      for (int n = 0; n < 5; ++n) qb[n] = Qb[c0 + n * P.sf] + acc[0][n];
+      const real Tb = Qb[c0 + 5 * P.sf];
+      finalize_cell(q[0], qb, Tb, va, sigma, extra, alpha, out, c0, P.sf, gm1);
+#pragma unroll
+      for (int n = 0; n < 6; ++n) Qb[c0 + n * P.sf] = (real)0;
+    } else {
+      flush(0);
+    }
+#pragma unroll
+    for (int n = 0; n < 5; ++n) va[n] = vb[n];
 #pragma unroll
     for (int o = 0; o < 3; ++o) {
       src[o] = src[o + 1];
@@ -1305,45 +1363,37 @@ __global__ void __launch_bounds__(256) k_adj_reduce(double* grad, long long ntil
   if (threadIdx.x == 0) { grad[0] += s[0][0]; grad[1] += s[1][0]; }
 }
 
-// out = sigma*v + J_c2p(U)^T Qb + alpha*extra; Qb cleared
+// out = sigma*v + J_c2p(U)^T Qb + alpha*extra; Qb cleared.  Fz > 0: the
+// z-face kernel (chunks of Fz faces) has finalized the cells z_exclusive
+// picks; they are skipped.
 template <typename real>
 __global__ void __launch_bounds__(256)
 k_adj_finalize(const real* __restrict__ U, const real* __restrict__ v, real sigma,
-               const real* __restrict__ extra, real alpha, real* Qb, real* out, DevParams P) {
+               const real* __restrict__ extra, real alpha, real* Qb, real* out, DevParams P,
+               int Fz) {
   const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (t >= P.sf) return;
+  if (Fz > 0 && z_exclusive((int)(t / P.sz), P.Nz, Fz)) return;
   const long long sf = P.sf;
   const real gm1 = (real)P.gm1;
   real q[5];
   load_prim_arr(U, t, sf, gm1, q);
-  real qb[5];
-#pragma unroll
-  for (int n = 0; n < 5; ++n) qb[n] = Qb[t + n * sf];
-  const real Tb = Qb[t + 5 * sf];
-  // T = p / rho
-  qb[4] += Tb / q[0];
-  qb[0] -= Tb * q[4] / (q[0] * q[0]);
-  // consToPrim (States.h:34-45) reversed
-  const real r = q[0], u = q[1], vv = q[2], w = q[3];
-  const real Ub1 = qb[1] / r - qb[4] * gm1 * u;
-  const real Ub2 = qb[2] / r - qb[4] * gm1 * vv;
-  const real Ub3 = qb[3] / r - qb[4] * gm1 * w;
-  const real Ub4 = qb[4] * gm1;
-  const real Ub0 = qb[0] - (qb[1] * u + qb[2] * vv + qb[3] * w) / r +
-                   qb[4] * gm1 * (real)0.5 * (u * u + vv * vv + w * w);
-  const real Ub[5] = {Ub0, Ub1, Ub2, Ub3, Ub4};
+  real qb[5], vc[5];
 #pragma unroll
   for (int n = 0; n < 5; ++n) {
-    real o = sigma * v[t + n * sf] + Ub[n];
-    if (extra) o += alpha * extra[t + n * sf];
-    out[t + n * sf] = o;
+    qb[n] = Qb[t + n * sf];
+    vc[n] = v[t + n * sf];
   }
+  const real Tb = Qb[t + 5 * sf];
+  finalize_cell(q, qb, Tb, vc, sigma, extra, alpha, out, t, sf, gm1);
 #pragma unroll
   for (int n = 0; n < 6; ++n) Qb[t + n * sf] = (real)0;
 }
 
+// Fz (z faces only): set to the chunk length when the z-face kernel also
+// finalizes the cells only it reaches (the default path; launched last)
 template <typename real, int DIR, int RECON, int RS>
-static hipError_t adj_faces_dir(const AdjLaunch<real>& L, hipStream_t s) {
+static hipError_t adj_faces_dir(const AdjLaunch<real>& L, hipStream_t s, int* Fz = nullptr) {
   const DevParams& P = L.P;
   if (L.simple) {
     const long long nf = (long long)(P.Nx + (DIR == FV3D_IX)) * (P.Ny + (DIR == FV3D_IY)) *
@@ -1358,8 +1408,14 @@ static hipError_t adj_faces_dir(const AdjLaunch<real>& L, hipStream_t s) {
     const int Nd = DIR == FV3D_IY ? P.Ny : P.Nz, Nt = DIR == FV3D_IY ? P.Nz : P.Ny;
     const int nch = (Nd + 1 + 15) / 16, F = (Nd + 1 + nch - 1) / nch;
     const dim3 g((P.Nx + 63) / 64, (Nt + AL_T - 1) / AL_T, (Nd + 1 + F - 1) / F);
-    hipLaunchKernelGGL((k_adj_faces_line<real, DIR == FV3D_IX ? FV3D_IY : DIR, RECON, RS>), g,
-                       dim3(64 * AL_T), 0, s, L.U, L.v, L.sigma, L.Qb, L.dt, P, F);
+    if (DIR == FV3D_IZ && Fz) {
+      hipLaunchKernelGGL((k_adj_faces_line<real, FV3D_IZ, RECON, RS, true>), g, dim3(64 * AL_T), 0, s,
+                         L.U, L.v, L.sigma, L.Qb, L.dt, P, F, L.extra, L.alpha, L.out);
+      *Fz = F;
+    } else {
+      hipLaunchKernelGGL((k_adj_faces_line<real, DIR == FV3D_IX ? FV3D_IY : DIR, RECON, RS>), g,
+                         dim3(64 * AL_T), 0, s, L.U, L.v, L.sigma, L.Qb, L.dt, P, F);
+    }
     return hipGetLastError();
   }
   dim3 grid;
@@ -1384,10 +1440,9 @@ static hipError_t adj_faces_dir(const AdjLaunch<real>& L, hipStream_t s) {
 }
 
 template <typename real, int RECON, int RS>
-static hipError_t adj_faces(const AdjLaunch<real>& L, hipStream_t s) {
+static hipError_t adj_faces_xy(const AdjLaunch<real>& L, hipStream_t s) {
   hipError_t e = adj_faces_dir<real, FV3D_IX, RECON, RS>(L, s);
   if (e == hipSuccess) e = adj_faces_dir<real, FV3D_IY, RECON, RS>(L, s);
-  if (e == hipSuccess) e = adj_faces_dir<real, FV3D_IZ, RECON, RS>(L, s);
   return e;
 }
 
@@ -1431,25 +1486,34 @@ static void adj_diffusion(const AdjLaunch<real>& L, hipStream_t s) {
   hipLaunchKernelGGL((k_adj_reduce<real>), dim3(1), dim3(256), 0, s, L.grad, ntl);
 }
 
-// one adjoint stage: Qb must be zero on entry and is zero on return
+// one adjoint stage: Qb must be zero on entry and is zero on return.  The
+// z faces go last, after the diffusion transpose, so that in the default
+// path they can finalize the cells only they still add to.
 template <typename real>
 static hipError_t launch_adjoint_stage_t(const AdjLaunch<real>& L, hipStream_t s) {
   const bool plm = L.recon == FV3D_PLM, hll = L.riemann == FV3D_HLL;
   hipError_t e;
-  if (plm && hll) e = adj_faces<real, FV3D_PLM, FV3D_HLL>(L, s);
-  else if (plm) e = adj_faces<real, FV3D_PLM, FV3D_HLLC>(L, s);
-  else if (hll) e = adj_faces<real, FV3D_PCM, FV3D_HLL>(L, s);
-  else e = adj_faces<real, FV3D_PCM, FV3D_HLLC>(L, s);
+  if (plm && hll) e = adj_faces_xy<real, FV3D_PLM, FV3D_HLL>(L, s);
+  else if (plm) e = adj_faces_xy<real, FV3D_PLM, FV3D_HLLC>(L, s);
+  else if (hll) e = adj_faces_xy<real, FV3D_PCM, FV3D_HLL>(L, s);
+  else e = adj_faces_xy<real, FV3D_PCM, FV3D_HLLC>(L, s);
   if (e != hipSuccess) return e;
   if (L.tc && L.visc) adj_diffusion<real, true, true>(L, s);
   else if (L.tc) adj_diffusion<real, true, false>(L, s);
   else if (L.visc) adj_diffusion<real, false, true>(L, s);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
+  int Fz = 0;
+  int* fz = (L.simple || L.colored || L.no_fused_finalize) ? nullptr : &Fz;
+  if (plm && hll) e = adj_faces_dir<real, FV3D_IZ, FV3D_PLM, FV3D_HLL>(L, s, fz);
+  else if (plm) e = adj_faces_dir<real, FV3D_IZ, FV3D_PLM, FV3D_HLLC>(L, s, fz);
+  else if (hll) e = adj_faces_dir<real, FV3D_IZ, FV3D_PCM, FV3D_HLL>(L, s, fz);
+  else e = adj_faces_dir<real, FV3D_IZ, FV3D_PCM, FV3D_HLLC>(L, s, fz);
+  if (e != hipSuccess) return e;
   const int bs = 256;
   const unsigned nb = (unsigned)((L.P.sf + bs - 1) / bs);
   hipLaunchKernelGGL((k_adj_finalize<real>), dim3(nb), dim3(bs), 0, s, L.U, L.v, L.sigma,
-                     L.extra, L.alpha, L.Qb, L.out, L.P);
+                     L.extra, L.alpha, L.Qb, L.out, L.P, Fz);
   return hipGetLastError();
 }
 

@@ -109,6 +109,9 @@ struct AdjLaunch {
   // each pass holds too few tiles to fill the GPU)
   bool colored = false;
   bool colored_diffusion = false;
+  // finalize every cell in k_adj_finalize instead of most of them in the
+  // z-face kernel (check path)
+  bool no_fused_finalize = false;
 };
 template <typename real>
 hipError_t launch_adjoint_stage(const AdjLaunch<real>& L, hipStream_t s);

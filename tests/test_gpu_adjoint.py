@@ -163,15 +163,20 @@ def test_adjoint_segment_store_does_not_change_result(gpu_available, name):
 @pytest.mark.parametrize("name", list(CONFIGS))
 def test_tiled_adjoint_equals_per_face_form(gpu_available, name, monkeypatch):
     """The LDS-tiled kernels against the one-thread-per-face/cell kernels
-    (FV3D_ADJ_SIMPLE): the same sums, associated differently."""
+    (FV3D_ADJ_SIMPLE): the same sums, associated differently.  The default
+    path finalizes most cells in the z-face kernel; FV3D_ADJ_UNFUSED leaves
+    them all to the finalize kernel."""
     pr = Problem(base_params(N=20, **CONFIGS[name]), nsteps=5)
     _, gt, kt, mt = pr.adjoint(K=2)
+    monkeypatch.setenv("FV3D_ADJ_UNFUSED", "1")
+    _, gu, ku, mu_ = pr.adjoint(K=2)
+    monkeypatch.delenv("FV3D_ADJ_UNFUSED")
     monkeypatch.setenv("FV3D_ADJ_COLORED", "1")  # colour passes instead of atomic flushes
     _, gc, kc, mc = pr.adjoint(K=2)
     monkeypatch.delenv("FV3D_ADJ_COLORED")
     monkeypatch.setenv("FV3D_ADJ_SIMPLE", "1")
     _, gs, ks, ms = pr.adjoint(K=2)
-    for g, k, m in [(gt, kt, mt), (gc, kc, mc)]:
+    for g, k, m in [(gt, kt, mt), (gu, ku, mu_), (gc, kc, mc)]:
         assert np.abs(g - gs).max() <= 1e-12 * np.abs(gs).max()
         assert k == pytest.approx(ks, rel=1e-12, abs=1e-300) and m == pytest.approx(ms, rel=1e-12, abs=1e-300)
 
