@@ -62,6 +62,24 @@ def read_traffic():
     return float(m.group(1)) if m else None
 
 
+# FP64 VALU lane-instructions MI355X issues per second: 256 CUs x 4 SIMDs x
+# 16 lanes x 2.4 GHz (one wave64 FP64 instruction per SIMD every 4 cycles)
+VALU_PEAK_TINST = 256 * 4 * 16 * 2.4e9 / 1e12
+
+
+def read_valu_per_cell():
+    """VALU lane-instructions per cell and stage of the FP64 blast stage
+    kernel, from the SQ_INSTS_VALU table of the committed rocprofv3 summary."""
+    try:
+        with open(TRAFFIC_SUMMARY) as f:
+            txt = f.read()
+    except OSError:
+        return None
+    m = re.search(r"\| `fv3d_hip::stage_kernel<double, 8, 2, 1, false, false, 0, 2>` \| [0-9.e+]+ \| ([0-9]+) \|",
+                  txt)
+    return float(m.group(1)) if m else None
+
+
 def cpu_baseline(n, budget_s):
     """The oracle (CPU restatement, reference pass structure) on the same
     grid: one warm-up step, then whole steps until budget_s has elapsed."""
@@ -428,6 +446,13 @@ def main():
             "mean_launch_ms": round(mean_launch_ms, 5),
         },
     }
+    valu = read_valu_per_cell()
+    if valu:
+        # the kernel is VALU-bound: its issue rate beside the HBM roofline
+        rate = valu * cells / (mean_launch_ms * 1e-3) / 1e12  # one launch = one stage over every cell
+        out["valu"] = {"lane_instr_per_cell_stage": valu, "achieved_Tinst_s": round(rate, 2),
+                       "peak_Tinst_s": round(VALU_PEAK_TINST, 2), "frac": round(rate / VALU_PEAK_TINST, 4),
+                       "source": "SQ_INSTS_VALU in profiles/stage_kernel_blast256_fp64_traffic.md"}
     if args.stream:
         out["roofline"]["stream_copy_GBs"] = round(stream_copy(), 1)
     if args.forms:
