@@ -31,6 +31,18 @@
  *                                   textbook stress; the z-branch index quirks
  *                                   (Viscosity.h:122-133) and the missing 1/d are
  *                                   pinned only by reading: parity unpinned
+ *   gravity, PCM_WB, WB wall flux   a discrete hydrostatic state stays at rest
+ *                                   (and moves without the shift or the wall
+ *                                   flux); sum(E - g rho z) is conserved
+ *   C91 walls                       no mass through the walls; ghosts mirror the
+ *                                   vertical momentum; the ghosts' pressure and
+ *                                   density extrapolation (with the gravity-flag
+ *                                   quirk) is pinned only by reading: parity
+ *                                   unpinned
+ *   B02 conductivity                one conductivity per face (energy conserved
+ *                                   between insulating walls); the tanh profile's
+ *                                   values and ComputeDt's integer-index use of it
+ *                                   are pinned only by reading: parity unpinned
  */
 #include <math.h>
 #include <stdlib.h>

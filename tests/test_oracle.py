@@ -431,3 +431,131 @@ def test_viscosity_shear_closed_forms():
     Unew = np.zeros_like(Q)
     o.viscosity(Q, Unew, dt)
     assert np.all(dom(p, Unew) == 0.0)
+
+
+# ---------------------------------------------------------------------------
+# gravity, the well-balanced variants, C91 walls and B02 conductivity
+# ---------------------------------------------------------------------------
+def _stratified(p, p0=2.0, seed=None, uniform_rho=False):
+    """rho(z) = 1 + 0.3 cos(2 pi z) (or 1); p from the discrete balance
+    p[k+1] = p[k] + g dz (rho[k] + rho[k+1]) / 2 that PCM_WB's face
+    pressures p +- rho g dz / 2 (Update.h:27) and the source +rho g
+    (Update.h:149) hold in equilibrium.  With `seed`, a random velocity
+    and pressure perturbation on top."""
+    Q = np.zeros(p.shape_ghosted)
+    d = dom(p, Q)
+    z = p.zmin + (np.arange(p.Nz) + 0.5) * p.dz
+    rho = np.ones(p.Nz) if uniform_rho else 1.0 + 0.3 * np.cos(2.0 * np.pi * z)
+    pr = np.empty(p.Nz)
+    pr[0] = p0
+    for k in range(p.Nz - 1):
+        pr[k + 1] = pr[k] + p.g * p.dz * (rho[k] + rho[k + 1]) * 0.5
+    d[..., 0] = rho[:, None, None]
+    d[..., 4] = pr[:, None, None]
+    if seed is not None:
+        rng = np.random.default_rng(seed)
+        d[..., 1:4] = 0.05 * rng.standard_normal(d[..., 1:4].shape)
+        d[..., 4] *= 1.0 + 0.02 * rng.standard_normal(d[..., 4].shape)
+    return Q
+
+
+def _gravity_params(**kw):
+    base = dict(Nx=6, Ny=5, Nz=16, boundary_x=F.BC_PERIODIC, boundary_y=F.BC_PERIODIC,
+                boundary_z=F.BC_REFLECTING, riemann_solver=F.HLLC, time_stepping=F.TS_RK2, CFL=0.4,
+                gravity=1, g=1.0, tend=1e9)
+    base.update(kw)
+    return Params.defaults().replace(**base)
+
+
+@pytest.mark.parametrize("solver", [F.HLL, F.HLLC])
+def test_well_balanced_hydrostatic_state_stays_at_rest(solver):
+    """PCM_WB with the well-balanced z-wall flux keeps a discrete hydrostatic
+    state at rest to round-off; this fixes the sign and size of PCM_WB's
+    pressure shift, the gravity momentum source and both wall overrides
+    (Update.h:132-137).  Without the shift (PCM), or without the wall flux,
+    the same state starts to move.  HLLC keeps the density steps of a
+    stratified state as stationary contacts; HLL smears any contact, so its
+    state has uniform density (a linear pressure)."""
+    def max_speed(**kw):
+        p = _gravity_params(riemann_solver=solver, **kw)
+        o = Oracle(p)
+        Q0 = _stratified(p, uniform_rho=solver == F.HLL)
+        U, _, _, _ = o.run(Q0, 10, stop_at_tend=False)
+        q = dom(p, o.cons_to_prim(U))
+        q0 = dom(p, Q0)
+        return (float(np.abs(q[..., 1:4]).max()),
+                float(np.abs(q[..., 0] / q0[..., 0] - 1).max()),
+                float(np.abs(q[..., 4] / q0[..., 4] - 1).max()))
+    w, drho, dp = max_speed(reconstruction=F.PCM_WB, well_balanced_flux_at_z_bc=1)
+    assert w < 1e-12 and drho < 1e-13 and dp < 1e-13, (w, drho, dp)
+    assert max_speed(reconstruction=F.PCM, well_balanced_flux_at_z_bc=1)[0] > 1e-6
+    assert max_speed(reconstruction=F.PCM_WB, well_balanced_flux_at_z_bc=0)[0] > 1e-6
+
+
+@pytest.mark.parametrize("recon", [F.PCM, F.PLM])
+def test_gravity_conserves_energy_with_potential(recon):
+    """The gravity source adds dt g (F_low + F_high)/2 of the z mass fluxes to
+    E (Update.h:149-151), which is the discrete work of the potential
+    -g rho z: between reflecting z walls, sum(E - g rho z) stays to
+    round-off while sum(E) itself moves.  A dropped or mis-weighted energy
+    source breaks the first sum."""
+    p = _gravity_params(reconstruction=recon, g=0.7)
+    o = Oracle(p)
+    Q0 = _stratified(p, seed=3)
+    z = p.zmin + (np.arange(p.Nz) + 0.5) * p.dz
+    def sums(U):
+        u = dom(p, U)
+        return float(u[..., 4].sum()), float((u[..., 4] - p.g * u[..., 0] * z[:, None, None]).sum())
+    U0 = o.prim_to_cons(Q0)
+    U, _, _, _ = o.run(Q0, 15, stop_at_tend=False)
+    (e0, t0), (e1, t1) = sums(U0), sums(U)
+    scale = float(np.abs(dom(p, U0)[..., 4]).sum())
+    assert abs(t1 - t0) < 1e-13 * scale, (t0, t1)
+    assert abs(e1 - e0) > 1e-6 * scale, (e0, e1)
+
+
+def test_c91_walls_conserve_mass_and_reflect_vertical_momentum():
+    """C91 z walls (BoundaryConditions.h:86-134, Update.h:138-143): the wall
+    face flux carries no mass, so total mass stays to round-off, and every
+    z ghost holds the mirrored cell's vertical momentum with the opposite
+    sign (rho_g w_g = -rho_s w_s) at both walls."""
+    p = _gravity_params(boundary_z=F.BC_C91, reconstruction=F.PLM, g=2.0, theta1=2.0, m1=1.0)
+    o = Oracle(p)
+    Q0 = F.problems.c91(p)
+    rng = np.random.default_rng(5)
+    d = dom(p, Q0)
+    d[..., 1:4] = 0.05 * rng.standard_normal(d[..., 1:4].shape)
+    U0 = o.prim_to_cons(Q0)
+    U, _, _, _ = o.run(Q0, 12, stop_at_tend=False)
+    m0, m1 = dom(p, U0)[..., 0].sum(), dom(p, U)[..., 0].sum()
+    assert abs(m1 - m0) < 1e-13 * m0, (m0, m1)
+    Q = o.fill_boundaries(Q0.copy())
+    kb, ke = p.kbeg, p.kend
+    for g, s in [(kb - 1, kb), (kb - 2, kb + 1), (ke, ke - 1), (ke + 1, ke - 2)]:
+        mg = Q[g, p.jbeg:p.jend, p.ibeg:p.iend, 0] * Q[g, p.jbeg:p.jend, p.ibeg:p.iend, 3]
+        ms = Q[s, p.jbeg:p.jend, p.ibeg:p.iend, 0] * Q[s, p.jbeg:p.jend, p.ibeg:p.iend, 3]
+        assert np.allclose(mg, -ms, rtol=1e-14, atol=1e-16), (g, s)
+
+
+def test_b02_conduction_conserves_energy_between_insulating_walls():
+    """conductivity_mode=B02 (ThermalConduction.h:7-22): each z face has one
+    conductivity, used by both its cells, so between reflecting walls with
+    no wall condition (mirrored T, zero wall flux) the total energy stays to
+    round-off while heat moves.  A profile evaluated per cell instead of per
+    face breaks the sum."""
+    p = Params.defaults().replace(Nx=4, Ny=4, Nz=24, boundary_x=F.BC_PERIODIC, boundary_y=F.BC_PERIODIC,
+                                  boundary_z=F.BC_REFLECTING, reconstruction=F.PLM, time_stepping=F.TS_RK2,
+                                  thermal_conductivity_active=1, thermal_conductivity_mode=F.TCM_B02,
+                                  kappa=0.05, b02_zmid=0.5, b02_kappa1=1.0, b02_kappa2=0.1,
+                                  b02_thickness=0.1, CFL=0.4, tend=1e9)
+    o = Oracle(p)
+    Q0 = np.zeros(p.shape_ghosted)
+    d = dom(p, Q0)
+    z = p.zmin + (np.arange(p.Nz) + 0.5) * p.dz
+    d[..., 0] = 1.0
+    d[..., 4] = (1.0 + 0.5 * np.sin(3.0 * np.pi * z))[:, None, None]  # T = p at rest
+    U0 = o.prim_to_cons(Q0)
+    U, _, _, _ = o.run(Q0, 10, stop_at_tend=False)
+    e0, e1 = dom(p, U0)[..., 4], dom(p, U)[..., 4]
+    assert abs(e1.sum() - e0.sum()) < 1e-13 * e0.sum()
+    assert np.abs(e1 - e0).max() > 1e-4
