@@ -1423,7 +1423,6 @@ static hipError_t adj_faces_dir(const AdjLaunch<real>& L, hipStream_t s, int* Fz
     return hipGetLastError();
   }
   if (DIR != FV3D_IX && !L.colored) {
-    // chunks of about 16 faces along the line
     const int Nd = DIR == FV3D_IY ? P.Ny : P.Nz, Nt = DIR == FV3D_IY ? P.Nz : P.Ny;
     // chunks of 15 to 32 faces, the count chosen so that the workgroups fill
     // whole rounds of the device (resident workgroups per CU x CUs): the
