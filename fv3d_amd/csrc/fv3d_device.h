@@ -261,7 +261,7 @@ __device__ __forceinline__ void unrotate_flux(real F[5]) {
 // at most negates one velocity, so the value of any ghost (edges and corners
 // included) is the cell reached by mapping each coordinate on its own, with
 // the normal velocity of every reflected axis negated.  C91 in x or y takes
-// the periodic branch of those passes (BoundaryConditions.h:373,394).
+// the periodic (default) branch of the X and Y passes (BoundaryConditions.h:165,186).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int map_coord(int i, int N, int bc, bool& flip) {
   flip = false;

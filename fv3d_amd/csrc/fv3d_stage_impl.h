@@ -48,7 +48,7 @@ __device__ __forceinline__ Prim<real> load_prim(const real* __restrict__ U, long
   const real u0 = U[base], u1 = U[base + sf], u2 = U[base + 2 * sf], u3 = U[base + 3 * sf],
              u4 = U[base + 4 * sf];
   Prim<real> q = cons_to_prim(u0, u1, u2, u3, u4, gm1);
-  // reflecting ghosts: BoundaryConditions.h:253-258 negates the normal velocity
+  // reflecting ghosts: BoundaryConditions.h:45-50 negates the normal velocity
   if (fx) q.u = -q.u;
   if (fy) q.v = -q.v;
   if (fz) q.w = -q.w;

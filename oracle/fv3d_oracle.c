@@ -18,9 +18,10 @@
  *   state prim<->cons, computeFlux  closed forms, round trips
  *   hll                             textbook HLL with Davis speeds; qL==qR;
  *                                   supersonic selection; Toro test states
- *   hllc                            textbook HLLC in 1D (v=w=0); qL==qR;
- *                                   the dropped-velocity quirk RiemannSolvers.h:52-55
- *                                   is pinned only by reading: parity unpinned
+ *   hllc                            textbook HLLC in 1D (v=w=0); qL==qR; the
+ *                                   dropped-velocity energy (RiemannSolvers.h:52-55)
+ *                                   as a closed form at qL==qR; the passive w is
+ *                                   upwinded and changes no other flux bit
  *   ghost fill                      brute-force index maps on tiny grids
  *   slopes                          hand cases, mirror antisymmetry
  *   flux update + RK2               exact Sod solution (L1 convergence),
@@ -28,21 +29,21 @@
  *   dt                              closed form at t=0
  *   conduction                      linear-T zero divergence, periodic conservation
  *   viscosity                       uniform flow -> 0; x/y branches against the
- *                                   textbook stress; the z-branch index quirks
- *                                   (Viscosity.h:122-133) and the missing 1/d are
- *                                   pinned only by reading: parity unpinned
+ *                                   textbook stress with the missing 1/d; the
+ *                                   z-branch index and component quirks
+ *                                   (Viscosity.h:122-129) by hand-derived closed
+ *                                   forms on bilinear fields
  *   gravity, PCM_WB, WB wall flux   a discrete hydrostatic state stays at rest
  *                                   (and moves without the shift or the wall
  *                                   flux); sum(E - g rho z) is conserved
- *   C91 walls                       no mass through the walls; ghosts mirror the
- *                                   vertical momentum; the ghosts' pressure and
- *                                   density extrapolation (with the gravity-flag
- *                                   quirk) is pinned only by reading: parity
- *                                   unpinned
+ *   C91 walls                       no mass through the walls; ghost values by
+ *                                   hand on a 4-level column (the gravity-flag
+ *                                   pressure of BoundaryConditions.h:104)
  *   B02 conductivity                one conductivity per face (energy conserved
- *                                   between insulating walls); the tanh profile's
- *                                   values and ComputeDt's integer-index use of it
- *                                   are pinned only by reading: parity unpinned
+ *                                   between insulating walls); kappa at zmid and
+ *                                   the tanh slope as closed forms; ComputeDt's
+ *                                   integer-index use (ComputeDt.h:39-41) as a
+ *                                   closed form of inv_dt_par_tc
  */
 #include <math.h>
 #include <stdlib.h>

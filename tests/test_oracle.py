@@ -559,3 +559,219 @@ def test_b02_conduction_conserves_energy_between_insulating_walls():
     e0, e1 = dom(p, U0)[..., 4], dom(p, U)[..., 4]
     assert abs(e1.sum() - e0.sum()) < 1e-13 * e0.sum()
     assert np.abs(e1 - e0).max() > 1e-4
+
+
+# ---------------------------------------------------------------------------
+# closed forms for the reference's quirks: each expected value below is
+# derived by hand from the field and the reference's stencil, and a textbook
+# (quirk-free) form of the same term gives a different number
+# ---------------------------------------------------------------------------
+def _visc_setup(field):
+    """Viscosity on a box with distinct power-of-two cell sizes (dx=1/8,
+    dy=1/16, dz=1/4); the primitive field is given on every cell, ghosts
+    included, so each interior cell sees the exact polynomial."""
+    p = Params.defaults().replace(Nx=8, Ny=16, Nz=4, boundary_x=F.BC_PERIODIC,
+                                  boundary_y=F.BC_PERIODIC, boundary_z=F.BC_PERIODIC,
+                                  viscosity_active=1, mu=0.3)
+    o = Oracle(p)
+    x = p.xmin + (np.arange(p.Ntx) - p.ibeg + 0.5) * p.dx
+    y = p.ymin + (np.arange(p.Nty) - p.jbeg + 0.5) * p.dy
+    z = p.zmin + (np.arange(p.Ntz) - p.kbeg + 0.5) * p.dz
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    Q = o.zeros()
+    Q[..., 0] = 1.0
+    Q[..., 4] = 1.0
+    for n, f in field(X, Y, Z).items():
+        Q[..., n] = f
+    Unew = o.zeros()
+    dt = 1.0
+    o.viscosity(Q, Unew, dt)
+    return p, dom(p, Unew), dom(p, X), dom(p, Y), dom(p, Z)
+
+
+def test_viscosity_z_branch_dudx_is_j_shifted():
+    """u = y.  The z-branch's dudx takes its minus terms from the j-shifted
+    column (Viscosity.h:122-123): dudx = -dy/(4dx) on the upper side and
+    +dy/(4dx) on the lower, so tau_zz differs between the sides and
+    d(rho w) = dt mu dy/(3 dx) (textbook: dudx = 0 and d(rho w) = 0).
+    Energy: the y-branch gives mu (qi_u(+) - qi_u(-)) = mu dy (no 1/dy)."""
+    p, d, X, Y, Z = _visc_setup(lambda X, Y, Z: {1: Y})
+    mu = p.mu
+    assert p.dy / (3 * p.dx) == pytest.approx(1.0 / 6.0)
+    assert np.allclose(d[..., 0], 0.0, atol=0) and np.allclose(d[..., 2], 0.0, atol=1e-15)
+    assert np.allclose(d[..., 1], 0.0, atol=1e-15)
+    assert np.allclose(d[..., 3], mu * p.dy / (3 * p.dx), rtol=1e-12)
+    assert np.allclose(d[..., 4], mu * p.dy, rtol=1e-12)
+
+
+def test_viscosity_z_branch_component_swaps_and_dz():
+    """v = x z.  The z-branch's dwdx reads v in its plus terms (Viscosity.h:124),
+    and dvdy / dwdy mix v and w and divide by dz (Viscosity.h:126-129).
+    Derived per side s (+: upper, -: lower), Z_s = z_{k+s} + z_k:
+      dwdx_s = x_{i+1} Z_s / (4 dx)  ->  d(rho u) = dt mu x_{i+1} dz / (2 dx)
+      dwdy_s = x_i Z_s / (4 dz), dvdz = x_i  ->  d(rho v) = dt mu x_i / 2
+      dvdy_s = -x_i Z_s / (4 dz)  ->  tau_zz jump, d(rho w) = dt mu x_i / 3
+      E: x-branch mu z^2 dx, z-branch mu x^2 (dz + z)
+    The textbook stress gives d(rho u) = d(rho v) = d(rho w) = 0 here, and a
+    1/dy in dvdy or dwdy would scale the v and w terms by dz/dy = 4."""
+    p, d, X, Y, Z = _visc_setup(lambda X, Y, Z: {2: X * Z})
+    mu = p.mu
+    assert np.allclose(d[..., 1], mu * (X + p.dx) * p.dz / (2 * p.dx), rtol=1e-12)
+    assert np.allclose(d[..., 2], mu * X / 2, rtol=1e-12)
+    assert np.allclose(d[..., 3], mu * X / 3, rtol=1e-12)
+    assert np.allclose(d[..., 4], mu * (Z ** 2 * p.dx + X ** 2 * (p.dz + Z)), rtol=1e-12)
+
+
+def test_viscosity_z_branch_dwdx_minus_terms():
+    """w = y z.  dwdx's minus terms read w at (k, j+s, i-1) instead of
+    (k+s, j, i-1) (Viscosity.h:124-125): dwdx_s = -z_k (y_{j+s} + y_j)/(4 dx),
+    so d(rho u) = -dt mu z dy / (2 dx) (textbook: 0).  rho v: the z-branch
+    dwdy_s = -y_{j-1} Z_s/(4 dz) gives -mu y_{j-1}/2, and the y-branch
+    tau_yy jump gives -2/3 mu dy."""
+    p, d, X, Y, Z = _visc_setup(lambda X, Y, Z: {3: Y * Z})
+    mu = p.mu
+    assert np.allclose(d[..., 1], -mu * Z * p.dy / (2 * p.dx), rtol=1e-12, atol=1e-15)
+    assert np.allclose(d[..., 2], mu * (-0.5 * (Y - p.dy) - 2.0 / 3.0 * p.dy), rtol=1e-12)
+
+
+def test_c91_ghost_values_closed_form():
+    """fillC91 (BoundaryConditions.h:86-134) with gravity on and g = 40 on a
+    4-level column (dz = 1/4, theta1 = 2).  The ghosts' pressure uses the
+    gravity flag (1) where g belongs (Q17).  By hand:
+      k=-1: z=-1/8, dz'=-1/4, T=1-1/4,   P=3-1/4*2=2.5, rho=2.5/0.75
+      k=-2: z=-3/8, dz'=-1/2, T=1-3/4,   P=3-1/2*2=2,   rho=8
+      k=Nz:   Tref=4, dz'=1/4, T=4.5, P=4+1/4=4.25, rho=4.25/4.5
+      k=Nz+1: dz'=1/2, T=5, P=4.5, rho=0.9
+    velocities: top copies u, v of the mirrored cell and w -> -w rho_s/rho;
+    bottom scales all three by rho_s/rho and negates w."""
+    p = Params.defaults().replace(Nx=2, Ny=2, Nz=4, boundary_x=F.BC_PERIODIC, boundary_y=F.BC_PERIODIC,
+                                  boundary_z=F.BC_C91, gravity=1, g=40.0, theta1=2.0)
+    o = Oracle(p)
+    Q = o.zeros()
+    lev = {0: [2.0, 0.1, -0.2, 0.3, 3.0], 1: [1.5, 0.4, 0.5, 0.6, 2.5],
+           2: [1.25, -0.7, 0.8, -0.9, 3.5], 3: [1.0, 0.25, -0.5, 0.75, 4.0]}
+    for k, q in lev.items():
+        Q[p.kbeg + k] = q
+    o.fill_boundaries(Q)
+    kb, ke = p.kbeg, p.kend
+    rs = lambda k: lev[k][0]
+
+    def check(kg, r, P, u, v, w):
+        g = Q[kg, p.jbeg:p.jend, p.ibeg:p.iend]
+        assert np.allclose(g[..., 0], r, rtol=1e-15) and np.allclose(g[..., 4], P, rtol=1e-15), kg
+        assert np.allclose(g[..., 1], u, rtol=1e-15) and np.allclose(g[..., 2], v, rtol=1e-15), kg
+        assert np.allclose(g[..., 3], w, rtol=1e-15), kg
+
+    r = 2.5 / 0.75
+    check(kb - 1, r, 2.5, 0.1, -0.2, -0.3 * rs(0) / r)
+    check(kb - 2, 8.0, 2.0, 0.4, 0.5, -0.6 * rs(1) / 8.0)
+    r = 4.25 / 4.5
+    check(ke, r, 4.25, 0.25 * rs(3) / r, -0.5 * rs(3) / r, -0.75 * rs(3) / r)
+    check(ke + 1, 0.9, 4.5, -0.7 * rs(2) / 0.9, 0.8 * rs(2) / 0.9, 0.9 * rs(2) / 0.9)
+    # without gravity the extrapolated pressure is the reference cell's
+    p0 = p.replace(gravity=0)
+    Q0 = o.zeros()
+    for k, q in lev.items():
+        Q0[p.kbeg + k] = q
+    Oracle(p0).fill_boundaries(Q0)
+    assert np.all(Q0[kb - 1, p.jbeg:p.jend, p.ibeg:p.iend, 4] == 3.0)
+    assert np.all(Q0[ke, p.jbeg:p.jend, p.ibeg:p.iend, 4] == 4.0)
+
+
+def _b02(**kw):
+    base = dict(Nx=8, Ny=4, Nz=8, boundary_x=F.BC_PERIODIC, boundary_y=F.BC_PERIODIC,
+                boundary_z=F.BC_PERIODIC, thermal_conductivity_active=1,
+                thermal_conductivity_mode=F.TCM_B02, kappa=0.05, b02_kappa1=1.0, b02_kappa2=3.0,
+                b02_thickness=0.2)
+    base.update(kw)
+    return Params.defaults().replace(**base)
+
+
+def _tc_delta(p, T):
+    """E change of one conduction pass with dt = 1 for a temperature field
+    T(x, y, z) given on every cell (rho = 1 at rest)."""
+    o = Oracle(p)
+    x = p.xmin + (np.arange(p.Ntx) - p.ibeg + 0.5) * p.dx
+    y = p.ymin + (np.arange(p.Nty) - p.jbeg + 0.5) * p.dy
+    z = p.zmin + (np.arange(p.Ntz) - p.kbeg + 0.5) * p.dz
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    Q = o.zeros()
+    Q[..., 0] = 1.0
+    Q[..., 4] = T(X, Y, Z)
+    Unew = o.zeros()
+    o.thermal_conduction(Q, Unew, 1.0)
+    return dom(p, Unew)[..., 4]
+
+
+def test_b02_profile_closed_forms():
+    """computeKappa's B02 profile (ThermalConduction.h:11-16):
+    kappa(z) = kappa (k1 (1-tr) + k2 tr), tr = (tanh((z - zmid)/th) + 1)/2.
+    With zmid on the centre of level 3:
+     * T = 1 + x^2: the x faces of level 3 carry kappa(zmid) = kappa (k1+k2)/2,
+       so dE = dt kappa (k1 + k2) (the discrete second difference is 2);
+     * T = 1 + z: dE = dt/dz (kB - kF) = dt/dz kappa (k2 - k1) tanh(dz/th) / 2
+       (the faces average the neighbouring centres' conductivities)."""
+    p = _b02()
+    zmid = p.zmin + 3.5 * p.dz
+    p = p.replace(b02_zmid=zmid)
+    d = _tc_delta(p, lambda X, Y, Z: 1.0 + X ** 2)
+    assert np.allclose(d[3, :, 1:-1], p.kappa * (p.b02_kappa1 + p.b02_kappa2), rtol=1e-12)
+    d = _tc_delta(p, lambda X, Y, Z: 1.0 + Z)
+    ex = p.kappa * (p.b02_kappa2 - p.b02_kappa1) * math.tanh(p.dz / p.b02_thickness) / (2 * p.dz)
+    assert np.allclose(d[3], ex, rtol=1e-12)
+    # the profile rises with z when k2 > k1: the level-5 x faces conduct more
+    d5 = _tc_delta(p, lambda X, Y, Z: 1.0 + X ** 2)[5, :, 1:-1]
+    assert np.all(d5 > p.kappa * (p.b02_kappa1 + p.b02_kappa2))
+
+
+def test_b02_compute_dt_uses_integer_indices():
+    """ComputeDt passes the ghosted integer indices (i, j, k) to computeKappa
+    as coordinates (ComputeDt.h:39-41).  With zmid = kend-1 (an index, far
+    outside the unit box) and k2 > k1 the largest conductivity is at the
+    last level, where tanh(0) = 0: inv_dt_par_tc = 2 kappa (k1+k2)/2 / d^2
+    for the smallest cell size d.  Physical coordinates would give about
+    2 kappa k1 / d^2."""
+    p = _b02(Nz=16)
+    p = p.replace(b02_zmid=float(p.kend - 1))
+    o = Oracle(p)
+    Q = o.zeros()
+    Q[..., 0] = 1.0
+    Q[..., 4] = 1.0
+    _, inv = o.compute_dt(Q)
+    dmin = min(p.dx, p.dy, p.dz)
+    assert inv[1] == pytest.approx(p.kappa * (p.b02_kappa1 + p.b02_kappa2) / dmin ** 2, rel=1e-14)
+
+
+def test_hllc_passive_w_is_upwinded():
+    """The second transverse velocity w only rides along (RiemannSolvers.h:52,
+    87, 96, 110): changing wL and wR leaves the mass, normal and first
+    transverse momentum and energy fluxes and pout bit for bit, and the w
+    flux is F_rho times w of the upwind side (left when SL > 0 or S* > 0)."""
+    g = 1.4
+    o = Oracle(Params.defaults().replace(gamma0=g))
+    qL, qR = _states(4000, 21), _states(4000, 22)
+    qL[:400, 1] += 20.0
+    qR[:400, 1] += 20.0
+    qL[400:800, 1] -= 20.0
+    qR[400:800, 1] -= 20.0
+    f1, p1 = o.riemann(F.HLLC, qL, qR)
+    rng = np.random.default_rng(23)
+    qL2, qR2 = qL.copy(), qR.copy()
+    qL2[:, 3] = rng.normal(0, 2, len(qL))
+    qR2[:, 3] = rng.normal(0, 2, len(qR))
+    f2, p2 = o.riemann(F.HLLC, qL2, qR2)
+    for n in (0, 1, 2, 4):
+        assert np.array_equal(f1[:, n], f2[:, n]), n
+    assert np.array_equal(p1, p2)
+    rL, uL, _, _, pL = qL.T
+    rR, uR, _, _, pR = qR.T
+    cL, cR = np.sqrt(g * pL / rL), np.sqrt(g * pR / rR)
+    SL = np.minimum(uL, uR) - np.maximum(cL, cR)
+    SR = np.maximum(uL, uR) + np.maximum(cL, cR)
+    Ss = (pR - pL + rL * uL * (SL - uL) - rR * uR * (SR - uR)) / (rL * (SL - uL) - rR * (SR - uR))
+    clear = np.abs(Ss) > 1e-9
+    left = (SL > 0) | (Ss > 0)
+    assert clear.sum() > 3900 and left[clear].any() and (~left[clear]).any()
+    for q1, q2, f in ((qL, qR, f1), (qL2, qR2, f2)):
+        w = np.where(left, q1[:, 3], q2[:, 3])
+        assert np.array_equal(f[clear, 3], (f[:, 0] * w)[clear])
