@@ -15,6 +15,7 @@ import ctypes
 import os
 import sys
 import warnings
+import weakref
 from dataclasses import dataclass
 from typing import Optional
 
@@ -165,7 +166,26 @@ def lib() -> ctypes.CDLL:
         "fv3d_get_cons": (ctypes.c_int, [vp, dp, ctypes.c_int]),
         "fv3d_compute_dt": (ctypes.c_int, [vp, dp, dp]),
         "fv3d_step": (ctypes.c_int, [vp]),
+        "fv3d_step_dt": (ctypes.c_int, [vp, ctypes.c_double]),
         "fv3d_steps": (ctypes.c_int, [vp, ctypes.c_int]),
+        "fv3d_array_create": (ctypes.c_int, [vp, P(vp)]),
+        "fv3d_array_destroy": (ctypes.c_int, [vp]),
+        "fv3d_array_data": (ctypes.c_void_p, [vp]),
+        "fv3d_array_upload": (ctypes.c_int, [vp, dp]),
+        "fv3d_array_download": (ctypes.c_int, [vp, dp]),
+        "fv3d_array_copy": (ctypes.c_int, [vp, vp]),
+        "fv3d_cons_to_prim_array": (ctypes.c_int, [vp, vp, vp]),
+        "fv3d_prim_to_cons_array": (ctypes.c_int, [vp, vp, vp]),
+        "fv3d_fill_boundaries_array": (ctypes.c_int, [vp, vp]),
+        "fv3d_compute_slopes": (ctypes.c_int, [vp, vp, vp, vp, vp]),
+        "fv3d_fluxes_and_update": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, ctypes.c_double]),
+        "fv3d_apply_tc": (ctypes.c_int, [vp, vp, vp, ctypes.c_double]),
+        "fv3d_apply_visc": (ctypes.c_int, [vp, vp, vp, ctypes.c_double]),
+        "fv3d_euler_step": (ctypes.c_int, [vp, vp, vp, ctypes.c_double]),
+        "fv3d_update": (ctypes.c_int, [vp, vp, vp, ctypes.c_double]),
+        "fv3d_compute_dt_array": (ctypes.c_int, [vp, vp, dp, dp]),
+        "fv3d_get_cons_array": (ctypes.c_int, [vp, vp]),
+        "fv3d_set_cons_array": (ctypes.c_int, [vp, vp]),
         "fv3d_step_count": (ctypes.c_long, [vp]),
         "fv3d_synchronize": (ctypes.c_int, [vp]),
         "fv3d_time_steps": (ctypes.c_int, [vp, ctypes.c_int, P(ctypes.c_float), P(ctypes.c_float)]),
@@ -232,10 +252,13 @@ class Simulation:
         self.params = params.copy()
         self.nonfinite = False
         self._ctx = ctypes.c_void_p()
+        self._arrays = weakref.WeakSet()
         _check(lib().fv3d_create(ctypes.byref(self.params), precision, device, ctypes.byref(self._ctx)))
 
     def close(self):
         if self._ctx:
+            for a in list(self._arrays):  # arrays die before their context
+                a.close()
             lib().fv3d_destroy(self._ctx)
             self._ctx = ctypes.c_void_p()
 
@@ -307,6 +330,69 @@ class Simulation:
     def step(self, n: int = 1):
         _check(lib().fv3d_steps(self._ctx, int(n)), self._ctx)
 
+    def step_dt(self, dt: float):
+        """UpdateFunctor::update(Q, Unew, dt) with the caller's dt (Update.h:180)."""
+        _check(lib().fv3d_step_dt(self._ctx, float(dt)), self._ctx)
+
+    # ---- the reference's functors one pass at a time, on device arrays ----
+    def array(self, host: Optional[np.ndarray] = None) -> "Array":
+        """A new device Array of this box (zero, or a copy of `host`)."""
+        a = Array(self)
+        if host is not None:
+            a.upload(host)
+        return a
+
+    def _fn(self, name, *args):
+        _check(getattr(lib(), name)(self._ctx, *args), self._ctx)
+
+    def cons_to_prim_array(self, U: "Array", Q: "Array"):
+        self._fn("fv3d_cons_to_prim_array", U._a, Q._a)
+
+    def prim_to_cons_array(self, Q: "Array", U: "Array"):
+        self._fn("fv3d_prim_to_cons_array", Q._a, U._a)
+
+    def fill_boundaries_array(self, Q: "Array"):
+        self._fn("fv3d_fill_boundaries_array", Q._a)
+
+    def compute_slopes(self, Q: "Array", sx: "Array", sy: "Array", sz: "Array"):
+        self._fn("fv3d_compute_slopes", Q._a, sx._a, sy._a, sz._a)
+
+    def fluxes_and_update(self, Q, sx, sy, sz, Unew, dt: float):
+        self._fn("fv3d_fluxes_and_update", Q._a, sx._a, sy._a, sz._a, Unew._a, float(dt))
+
+    def apply_tc(self, Q: "Array", Unew: "Array", dt: float):
+        self._fn("fv3d_apply_tc", Q._a, Unew._a, float(dt))
+
+    def apply_visc(self, Q: "Array", Unew: "Array", dt: float):
+        self._fn("fv3d_apply_visc", Q._a, Unew._a, float(dt))
+
+    def euler_step(self, Q: "Array", Unew: "Array", dt: float):
+        self._fn("fv3d_euler_step", Q._a, Unew._a, float(dt))
+
+    def update(self, Q: "Array", Unew: "Array", dt: float):
+        self._fn("fv3d_update", Q._a, Unew._a, float(dt))
+
+    def compute_dt_array(self, Q: "Array", strict: bool = False):
+        """(dt, (inv_dt_hyp, inv_dt_par_tc, inv_dt_par_visc)) of a primitive
+        array.  Non-finite cells are left out of the maximum with a
+        NonFiniteWarning, or raise with ``strict``, as in compute_dt."""
+        inv = np.zeros(3)
+        dt = ctypes.c_double()
+        rc = lib().fv3d_compute_dt_array(self._ctx, Q._a, _dptr(inv), ctypes.byref(dt))
+        if rc == ENONFINITE and not strict:
+            warnings.warn("fv3d: a cell of the array has a non-finite inverse time step; it is "
+                          "left out of dt, as in the reference", NonFiniteWarning, stacklevel=2)
+        else:
+            _check(rc, self._ctx)
+        return dt.value, tuple(inv)
+
+    def get_cons_array(self, U: "Array"):
+        self._fn("fv3d_get_cons_array", U._a)
+
+    def set_cons_array(self, U: "Array"):
+        self._fn("fv3d_set_cons_array", U._a)
+        self.nonfinite = False
+
     @property
     def step_count(self) -> int:
         return int(lib().fv3d_step_count(self._ctx))
@@ -357,6 +443,43 @@ class Simulation:
         ms = (ctypes.c_float * 3)()
         _check(lib().fv3d_adjoint_times(self._ctx, ms), self._ctx)
         return tuple(ms)
+
+
+class Array:
+    """One Array of the reference (Kokkos::View<double****> of extents
+    (Ntz, Nty, Ntx, 5), SimInfo.h:14) in device memory, owned by a
+    Simulation; upload/download use that host layout."""
+
+    def __init__(self, sim: Simulation):
+        self.sim = sim
+        self._a = ctypes.c_void_p()
+        _check(lib().fv3d_array_create(sim._ctx, ctypes.byref(self._a)), sim._ctx)
+        sim._arrays.add(self)
+
+    def upload(self, host: np.ndarray):
+        host = np.ascontiguousarray(host, dtype=np.float64)
+        if host.shape != self.sim.params.shape_ghosted:
+            raise ValueError(f"shape {host.shape}, expected {self.sim.params.shape_ghosted}")
+        _check(lib().fv3d_array_upload(self._a, _dptr(host)), self.sim._ctx)
+
+    def download(self) -> np.ndarray:
+        out = np.zeros(self.sim.params.shape_ghosted)
+        _check(lib().fv3d_array_download(self._a, _dptr(out)), self.sim._ctx)
+        return out
+
+    def copy_from(self, other: "Array"):
+        _check(lib().fv3d_array_copy(self._a, other._a), self.sim._ctx)
+
+    def close(self):
+        if self._a:
+            lib().fv3d_array_destroy(self._a)
+            self._a = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 BATCH_GRAPH, BATCH_PERSISTENT, BATCH_LOOP = 0, 1, 2

@@ -29,6 +29,7 @@
 #pragma once
 
 #include <algorithm>
+#include <atomic>
 #include <type_traits>
 
 #include "fv3d_device.h"
@@ -1390,21 +1391,26 @@ k_adj_finalize(const real* __restrict__ U, const real* __restrict__ v, real sigm
   for (int n = 0; n < 6; ++n) Qb[t + n * sf] = (real)0;
 }
 
-// resident workgroups of a line kernel instance on the whole device,
-// queried once
+// resident workgroups of a line kernel instance on the whole current
+// device, queried once per device (threads that race here compute the same
+// value; the cache is atomic)
 template <typename real, int DIR, int RECON, int RS, bool FIN>
 static long long line_round_capacity() {
-  static long long cap = 0;
+  constexpr int kMaxDev = 64;
+  static std::atomic<long long> caps[kMaxDev];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+  long long cap = dev >= 0 && dev < kMaxDev ? caps[dev].load(std::memory_order_relaxed) : 0;
   if (cap == 0) {
-    int per_cu = 0, dev = 0, cus = 0;
+    int per_cu = 0, cus = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &per_cu, reinterpret_cast<const void*>(&k_adj_faces_line<real, DIR, RECON, RS, FIN>),
             64 * AL_T, 0) != hipSuccess || per_cu < 1)
       per_cu = 1;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
       cus = 256;
     cap = (long long)per_cu * cus;
+    if (dev >= 0 && dev < kMaxDev) caps[dev].store(cap, std::memory_order_relaxed);
   }
   return cap;
 }

@@ -68,7 +68,7 @@ int fv3d_create(const fv3d_params* p, int precision, int device, fv3d_ctx** out)
   c->p = *p;
   c->precision = precision;
   c->device = device;
-  c->dp = make_devparams(*p);
+  c->dp = make_devparams(*p, 0, device_cus(device));
   c->elem = precision == FV3D_FP64 ? sizeof(double) : sizeof(float);
   c->state_bytes = (size_t)5 * p->Nx * p->Ny * p->Nz * c->elem;
   auto cleanup = [&](int code, const std::string& m) {
@@ -120,6 +120,9 @@ int fv3d_destroy(fv3d_ctx* c) {
   if (c->qbuf) hipFree(c->qbuf);
   if (c->c91g) hipFree(c->c91g);
   if (c->kztab) hipFree(c->kztab);
+  for (void* a : c->fn_arrays)
+    if (a) hipFree(a);
+  if (c->fn_acc) hipFree(c->fn_acc);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return FV3D_OK;
@@ -271,8 +274,10 @@ int fv3d_compute_dt(fv3d_ctx* c, double inv[3], double* dt) {
 
 }  // extern "C"
 
+// dt_slot: nullptr takes dt from the ring (CFL / max of the inverse time
+// steps of the current state); otherwise the device word holds dt itself
 template <typename real>
-static int launch_step_t(fv3d_ctx* c, hipEvent_t* ev) {
+static int launch_step_t(fv3d_ctx* c, hipEvent_t* ev, const unsigned long long* dt_slot) {
   const long s = c->step;
   unsigned long long* cur = c->ring + (s % 3);
   unsigned long long* nxt = c->ring + ((s + 1) % 3);
@@ -285,6 +290,10 @@ static int launch_step_t(fv3d_ctx* c, hipEvent_t* ev) {
   L.health = c->health;
   L.P = c->dp;
   L.inv_cur = cur;
+  if (dt_slot) {
+    L.inv_cur = dt_slot;
+    L.P.dt_direct = 1;
+  }
   real* A = (real*)c->A;
   real* B = (real*)c->B;
   if (kahan(c)) {
@@ -319,8 +328,9 @@ static int launch_step_t(fv3d_ctx* c, hipEvent_t* ev) {
   return FV3D_OK;
 }
 
-static int launch_step(fv3d_ctx* c, hipEvent_t* ev) {
-  return c->precision == FV3D_FP64 ? launch_step_t<double>(c, ev) : launch_step_t<float>(c, ev);
+static int launch_step(fv3d_ctx* c, hipEvent_t* ev, const unsigned long long* dt_slot = nullptr) {
+  return c->precision == FV3D_FP64 ? launch_step_t<double>(c, ev, dt_slot)
+                                   : launch_step_t<float>(c, ev, dt_slot);
 }
 
 extern "C" {
@@ -330,6 +340,19 @@ int fv3d_step(fv3d_ctx* c) {
   if (!c->have_state) return fail(c, FV3D_EINVAL, "no state set");
   HIPCHK(c, hipSetDevice(c->device));
   return launch_step(c, nullptr);
+}
+
+int fv3d_step_dt(fv3d_ctx* c, double dt) {
+  if (!c) return fail(c, FV3D_EINVAL, "null argument");
+  if (!c->have_state) return fail(c, FV3D_EINVAL, "no state set");
+  if (!std::isfinite(dt)) return fail(c, FV3D_EINVAL, "dt must be finite");
+  HIPCHK(c, hipSetDevice(c->device));
+  // ring slot 3 is not part of the three-slot inv_dt_hyp ring: it carries
+  // the caller's dt to both stages, stream-ordered before them
+  unsigned long long bits;
+  std::memcpy(&bits, &dt, sizeof bits);
+  HIPCHK(c, launch_put_u64(c->ring + 3, bits, c->stream));
+  return launch_step(c, nullptr, c->ring + 3);
 }
 
 int fv3d_steps(fv3d_ctx* c, int nsteps) {

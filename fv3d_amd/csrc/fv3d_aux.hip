@@ -285,6 +285,13 @@ __global__ void k_zero_u64(unsigned long long* p, int n) {
   if (i < n) p[i] = 0ull;
 }
 
+__global__ void k_put_one_u64(unsigned long long* p, unsigned long long v) { *p = v; }
+
+hipError_t launch_put_u64(unsigned long long* p, unsigned long long v, hipStream_t s) {
+  hipLaunchKernelGGL(k_put_one_u64, dim3(1), dim3(1), 0, s, p, v);
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------

@@ -28,6 +28,11 @@ struct fv3d_ctx {
   bool have_state = false;
   bool have_stage = false;  // B holds the reference's post-update Q source
   fv3d_hip::DevParams dp;
+  // the functor-level array path (fv3d_functors.hip), allocated on first
+  // use: slopes x/y/z, RK2's U0 and Ustar (ghosted FP64 arrays), and its dt
+  // accumulator
+  void* fn_arrays[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  unsigned long long* fn_acc = nullptr;
   float adj_ms[3] = {0, 0, 0};  // last adjoint run: forward pass, backward pass, total
   size_t adj_store_bytes = 0;   // device memory for segments kept from the forward pass
   std::string err;

@@ -31,16 +31,17 @@ inline double inv_dt_par(bool active, double coef, const fv3d_params& p) {
 }
 
 // z-chunk length of the stage kernel.  One stage workgroup fits on a CU
-// (LDS and VGPRs), so tiles_xy * tz workgroups run in ceil(tiles_xy*tz/256)
-// rounds on MI355X's 256 CUs, each marching a chunk of nzc planes after a
-// warm-up worth about 1.5 planes (measured on the 256^3 blast: 1, 2 and 4
-// rounds of 128, 64 and 32 planes took 1.588, 1.607 and 1.630 ms per step).
-// The chunk count minimises rounds x (nzc + 1.5); FV3D_NZC overrides.
-inline int tiles_z_for(const fv3d_params& p, int tiles_xy) {
+// (LDS and VGPRs), so tiles_xy * tz workgroups run in
+// ceil(tiles_xy*tz/cus) rounds (256 CUs on MI355X), each marching a chunk of
+// nzc planes after a warm-up worth about 1.5 planes (measured on the 256^3
+// blast: 1, 2 and 4 rounds of 128, 64 and 32 planes took 1.588, 1.607 and
+// 1.630 ms per step).  The chunk count minimises rounds x (nzc + 1.5);
+// FV3D_NZC overrides.
+inline int tiles_z_for(const fv3d_params& p, int tiles_xy, int cus) {
   int nzc = 0;
   if (const char* e = std::getenv("FV3D_NZC")) nzc = std::atoi(e);
   if (nzc <= 0) {
-    constexpr int kCUs = 256;
+    const int kCUs = cus > 0 ? cus : 256;
     double best = 0.0;
     for (int tz = 1; tz <= std::min(p.Nz, 64); ++tz) {
       const int n = (p.Nz + tz - 1) / tz;
@@ -89,7 +90,17 @@ inline int check_params(const fv3d_params& p, std::string& why) {
   return 0;
 }
 
-inline DevParams make_devparams(const fv3d_params& p, int nzc_override = 0) {
+// the compute units of a device (256 on MI355X; fewer on a partitioned one)
+inline int device_cus(int device) {
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+      cus < 1)
+    cus = 256;
+  return cus;
+}
+
+// kernel parameters of a box; cus sizes the z chunks (tiles_z_for)
+inline DevParams make_devparams(const fv3d_params& p, int nzc_override, int cus) {
   DevParams d;
   std::memset(&d, 0, sizeof d);
   d.Nx = p.Nx; d.Ny = p.Ny; d.Nz = p.Nz;
@@ -133,7 +144,8 @@ inline DevParams make_devparams(const fv3d_params& p, int nzc_override = 0) {
   d.tiles_x = (p.Nx + STAGE_WX - 1) / STAGE_WX;
   const int ty = p.viscosity_active ? STAGE_TY_VISC : STAGE_TY;
   d.tiles_y = (p.Ny + ty - 1) / ty;
-  d.nzc = nzc_override > 0 ? std::max(1, std::min(nzc_override, p.Nz)) : tiles_z_for(p, d.tiles_x * d.tiles_y);
+  d.nzc = nzc_override > 0 ? std::max(1, std::min(nzc_override, p.Nz))
+                           : tiles_z_for(p, d.tiles_x * d.tiles_y, cus);
   d.tiles_z = (p.Nz + d.nzc - 1) / d.nzc;
   return d;
 }

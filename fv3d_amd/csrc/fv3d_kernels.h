@@ -136,5 +136,7 @@ hipError_t launch_riemann_batch(int solver, double gamma, long n, const double* 
 // plain device copy of n doubles (n even)
 hipError_t launch_copy(const double* a, double* b, long long n, hipStream_t s);
 hipError_t launch_fill_zero_u64(unsigned long long* p, int n, hipStream_t s);
+// *p = v, stream-ordered (a one-thread kernel, safe under graph capture)
+hipError_t launch_put_u64(unsigned long long* p, unsigned long long v, hipStream_t s);
 
 }  // namespace fv3d_hip

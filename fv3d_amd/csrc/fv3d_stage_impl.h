@@ -26,6 +26,7 @@
 #pragma once
 #include "fv3d_device.h"
 #include "fv3d_kernels.h"
+#include "fv3d_physics.h"
 
 namespace fv3d_hip {
 
@@ -399,14 +400,7 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
       }
       u[0] = fmax((real)1.0e-6, u[0]);  // Update.h:160
 
-      if (TC) {  // ThermalConduction.h:47-95, constant conductivity
-        const real TC_ = T0;
-        const real TL = sT[r][c - 1], TR = sT[r][c + 1];
-        const real TU = sT[r - 1][c], TD = sT[r + 1][c];
-        const real TF = Tm, TB = T1;
-        auto ddx = [&](real v) { return over_h(v, p2x, P.idx, P.dx); };
-        auto ddy = [&](real v) { return over_h(v, p2y, P.idy, P.dy); };
-        auto ddz = [&](real v) { return over_h(v, p2z, P.idz, P.dz); };
+      if (TC) {  // ThermalConduction.h:47-95
         // face conductivities: constant, or B02's z profile (kappaL..kappaB)
         real kxy = kf, kzF = kf, kzB = kf;
         if (kztab) {
@@ -414,25 +408,10 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
           kzF = (real)kztab[P.Nz + k];
           kzB = (real)kztab[2 * P.Nz + k];
         }
-        const real FL = ddx(kxy * (TC_ - TL));
-        const real FR = ddx(kxy * (TR - TC_));
-        const real FU = ddy(kxy * (TC_ - TU));
-        const real FD = ddy(kxy * (TD - TC_));
-        real FF = ddz(kzF * (TC_ - TF));
-        real FB = ddz(kzB * (TB - TC_));
-        if (k == 0 && P.bctc_zmin != FV3D_BCTC_NONE) {
-          if (P.bctc_zmin == FV3D_BCTC_FIXED_TEMPERATURE)
-            FF = ddz(kzF * (real)2.0 * (TC_ - (real)P.bctc_zmin_value));
-          else if (P.bctc_zmin == FV3D_BCTC_FIXED_GRADIENT)
-            FF = kzF * (real)P.bctc_zmin_value;
-        }
-        if (k == P.Nz - 1 && P.bctc_zmax != FV3D_BCTC_NONE) {
-          if (P.bctc_zmax == FV3D_BCTC_FIXED_TEMPERATURE)
-            FB = ddz(kzB * (real)2.0 * ((real)P.bctc_zmax_value - TC_));
-          else if (P.bctc_zmax == FV3D_BCTC_FIXED_GRADIENT)
-            FB = kzB * (real)P.bctc_zmax_value;
-        }
-        u[4] = u[4] + (dtdx * (FR - FL) + dtdy * (FD - FU) + dtdz * (FB - FF));
+        u[4] = u[4] + conduction_increment<real>(T0, sT[r][c - 1], sT[r][c + 1], sT[r - 1][c],
+                                                 sT[r + 1][c], Tm, T1, kxy, kzF, kzB, k == 0,
+                                                 k == P.Nz - 1, P, p2x, p2y, p2z, dtdx, dtdy,
+                                                 dtdz);
       }
 
       if (VISC) {  // Viscosity.h:28-153, stencil[dk][dj][di] of u, v, w
@@ -442,82 +421,8 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
           if (dk == 0) return svm[v - 1][rr][cc];
           return svp[v - 1][rr][cc];
         };
-        const real four_thirds = (real)(4.0 / 3.0);
-        const real two_thirds = (real)(2.0 / 3.0);
         real vx[5] = {0, 0, 0, 0, 0}, vy[5] = {0, 0, 0, 0, 0}, vz[5] = {0, 0, 0, 0, 0};
-#pragma unroll
-        for (int s = 1; s < 3; ++s) {
-          const real sgn = (s == 1 ? (real)-1.0 : (real)1.0);
-          {  // X
-            const real qiu = (real)0.5 * (S(1, 1, s, 1) + S(1, 1, s - 1, 1));
-            const real qiv = (real)0.5 * (S(1, 1, s, 2) + S(1, 1, s - 1, 2));
-            const real qiw = (real)0.5 * (S(1, 1, s, 3) + S(1, 1, s - 1, 3));
-            const real dudx = oodx * (S(1, 1, s, 1) - S(1, 1, s - 1, 1));
-            const real dvdx = oodx * (S(1, 1, s, 2) - S(1, 1, s - 1, 2));
-            const real dwdx = oodx * (S(1, 1, s, 3) - S(1, 1, s - 1, 3));
-            const real dudy = (real)0.25 * oody * (S(1, 2, s, 1) - S(1, 0, s, 1)
-                                                   + S(1, 2, s - 1, 1) - S(1, 0, s - 1, 1));
-            const real dvdy = (real)0.25 * oody * (S(1, 2, s, 2) - S(1, 0, s, 2)
-                                                   + S(1, 2, s - 1, 2) - S(1, 0, s - 1, 2));
-            const real dudz = (real)0.25 * oodz * (S(2, 1, s, 1) - S(0, 1, s, 1)
-                                                   + S(2, 1, s - 1, 1) - S(0, 1, s - 1, 1));
-            const real dwdz = (real)0.25 * oodz * (S(2, 1, s, 3) - S(0, 1, s, 3)
-                                                   + S(2, 1, s - 1, 3) - S(0, 1, s - 1, 3));
-            const real txx = four_thirds * dudx - two_thirds * (dvdy + dwdz);
-            const real txy = dvdx + dudy;
-            const real txz = dwdx + dudz;
-            vx[1] += sgn * mu * txx;
-            vx[2] += sgn * mu * txy;
-            vx[3] += sgn * mu * txz;
-            vx[4] += sgn * mu * (txx * qiu + txy * qiv + txz * qiw);
-          }
-          {  // Y
-            const real qiu = (real)0.5 * (S(1, s, 1, 1) + S(1, s - 1, 1, 1));
-            const real qiv = (real)0.5 * (S(1, s, 1, 2) + S(1, s - 1, 1, 2));
-            const real qiw = (real)0.5 * (S(1, s, 1, 3) + S(1, s - 1, 1, 3));
-            const real dudy = oody * (S(1, s, 1, 1) - S(1, s - 1, 1, 1));
-            const real dvdy = oody * (S(1, s, 1, 2) - S(1, s - 1, 1, 2));
-            const real dwdy = oody * (S(1, s, 1, 3) - S(1, s - 1, 1, 3));
-            const real dudx = (real)0.25 * oodx * (S(1, s, 2, 1) - S(1, s, 0, 1)
-                                                   + S(1, s - 1, 2, 1) - S(1, s - 1, 0, 1));
-            const real dvdx = (real)0.25 * oodx * (S(1, s, 2, 2) - S(1, s, 0, 2)
-                                                   + S(1, s - 1, 2, 2) - S(1, s - 1, 0, 2));
-            const real dwdz = (real)0.25 * oodz * (S(2, s, 1, 3) - S(0, s, 1, 3)
-                                                   + S(2, s - 1, 1, 3) - S(0, s - 1, 1, 3));
-            const real dvdz = (real)0.25 * oodz * (S(2, s, 1, 2) - S(0, s, 1, 2)
-                                                   + S(2, s - 1, 1, 2) - S(0, s - 1, 1, 2));
-            const real tyy = four_thirds * dvdy - two_thirds * (dudx + dwdz);
-            const real txy = dvdx + dudy;
-            const real tyz = dvdz + dwdy;
-            vy[1] += sgn * mu * txy;
-            vy[2] += sgn * mu * tyy;
-            vy[3] += sgn * mu * tyz;
-            vy[4] += sgn * mu * (txy * qiu + tyy * qiv + tyz * qiw);
-          }
-          {  // Z, with the reference's index and component mix-ups (Viscosity.h:122-133)
-            const real qiu = (real)0.5 * (S(s, 1, 1, 1) + S(s - 1, 1, 1, 1));
-            const real qiv = (real)0.5 * (S(s, 1, 1, 2) + S(s - 1, 1, 1, 2));
-            const real qiw = (real)0.5 * (S(s, 1, 1, 3) + S(s - 1, 1, 1, 3));
-            const real dudz = oodz * (S(s, 1, 1, 1) - S(s - 1, 1, 1, 1));
-            const real dvdz = oodz * (S(s, 1, 1, 2) - S(s - 1, 1, 1, 2));
-            const real dwdz = oodz * (S(s, 1, 1, 3) - S(s - 1, 1, 1, 3));
-            const real dudx = (real)0.25 * oodx * (S(s, 1, 2, 1) - S(1, s, 0, 1)
-                                                   + S(s - 1, 1, 2, 1) - S(1, s - 1, 0, 1));
-            const real dwdx = (real)0.25 * oodx * (S(s, 1, 2, 2) - S(1, s, 0, 3)
-                                                   + S(s - 1, 1, 2, 2) - S(1, s - 1, 0, 3));
-            const real dvdy = (real)0.25 * oodz * (S(s, 2, 1, 3) - S(s, 0, 1, 2)
-                                                   + S(s - 1, 2, 1, 3) - S(s - 1, 0, 1, 2));
-            const real dwdy = (real)0.25 * oodz * (S(s, 2, 1, 2) - S(s, 0, 1, 3)
-                                                   + S(s - 1, 2, 1, 2) - S(s - 1, 0, 1, 3));
-            const real tzz = four_thirds * dwdz - two_thirds * (dudx + dvdy);
-            const real txz = dudz + dwdx;
-            const real tyz = dvdz + dwdy;
-            vz[1] += sgn * mu * txz;
-            vz[2] += sgn * mu * tyz;
-            vz[3] += sgn * mu * tzz;
-            vz[4] += sgn * mu * (txz * qiu + tyz * qiv + tzz * qiw);
-          }
-        }
+        viscous_fluxes<real>(S, oodx, oody, oodz, mu, vx, vy, vz);
 #pragma unroll
         for (int n = 1; n < 5; ++n) u[n] = u[n] + (vx[n] + vy[n] + vz[n]) * dt;
       }

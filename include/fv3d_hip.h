@@ -80,6 +80,12 @@ int fv3d_compute_dt(fv3d_ctx* ctx, double inv[3], double* dt);
  * device (no host synchronisation); read it with fv3d_compute_dt before the
  * step as main.cpp:57 does. */
 int fv3d_step(fv3d_ctx* ctx);
+/* UpdateFunctor::update(Q, Unew, dt) (Update.h:180-206) with the caller's
+ * dt instead of the device-side computeDt value: one Euler or SSP-RK2 step
+ * of the fused path.  The step after it reads dt from the ring again (the
+ * state this step produced reduces it, as after fv3d_step).  dt must be
+ * finite (FV3D_EINVAL otherwise). */
+int fv3d_step_dt(fv3d_ctx* ctx, double dt);
 /* nsteps steps back to back (no host synchronisation in between). */
 int fv3d_steps(fv3d_ctx* ctx, int nsteps);
 /* The number of steps taken since fv3d_set_prim / fv3d_set_cons. */
@@ -92,6 +98,67 @@ int fv3d_synchronize(fv3d_ctx* ctx);
  * Euler; may be NULL). */
 int fv3d_time_steps(fv3d_ctx* ctx, int nsteps, float* total_ms, float* stage_ms);
 
+/* ---- the reference's functors one pass at a time, on device arrays ----
+ *
+ * An fv3d_array is one Array of the reference (Kokkos::View<double****> of
+ * extents (Ntz, Nty, Ntx, 5), SimInfo.h:14) in device memory: FP64 with
+ * ghosts, stored variable-major [5][Ntz][Nty][Ntx] with x fastest.  It
+ * belongs to the context it was made for, uses that context's stream and
+ * is zero when created, like a new View.  Each entry point below is one
+ * pass of the reference over the same index range (range_tot, range_dom,
+ * range_slopes; SimInfo.h:298-303), one kernel per pass, so a caller that
+ * drives the functors one by one gets the reference's values pass for pass.
+ * fv3d_step is the fused form of the same arithmetic.  Calls are
+ * stream-ordered; only upload, download and fv3d_compute_dt_array wait. */
+typedef struct fv3d_array fv3d_array;
+
+int fv3d_array_create(fv3d_ctx* ctx, fv3d_array** out);
+int fv3d_array_destroy(fv3d_array* a);
+/* device address of the data, [5][Ntz][Nty][Ntx] doubles */
+double* fv3d_array_data(const fv3d_array* a);
+/* host array in the reference's host layout (Ntz, Nty, Ntx, 5) <-> device */
+int fv3d_array_upload(fv3d_array* a, const double* host);
+int fv3d_array_download(const fv3d_array* a, double* host);
+/* Kokkos::deep_copy(dst, src) */
+int fv3d_array_copy(fv3d_array* dst, const fv3d_array* src);
+
+/* consToPrim(U, Q) / primToCons(Q, U) over every cell, ghosts included
+ * (SimInfo.h:313-331) */
+int fv3d_cons_to_prim_array(fv3d_ctx* ctx, const fv3d_array* U, fv3d_array* Q);
+int fv3d_prim_to_cons_array(fv3d_ctx* ctx, const fv3d_array* Q, fv3d_array* U);
+/* BoundaryManager::fillBoundaries(Q) in place (BoundaryConditions.h:146-215) */
+int fv3d_fill_boundaries_array(fv3d_ctx* ctx, fv3d_array* Q);
+/* UpdateFunctor::computeSlopes(Q) (Update.h:55-88) into sx, sy, sz */
+int fv3d_compute_slopes(fv3d_ctx* ctx, const fv3d_array* Q, fv3d_array* sx, fv3d_array* sy,
+                        fv3d_array* sz);
+/* UpdateFunctor::computeFluxesAndUpdate(Q, Unew, dt) (Update.h:90-162)
+ * with the given slopes */
+int fv3d_fluxes_and_update(fv3d_ctx* ctx, const fv3d_array* Q, const fv3d_array* sx,
+                           const fv3d_array* sy, const fv3d_array* sz, fv3d_array* Unew,
+                           double dt);
+/* ThermalConductionFunctor::applyThermalConduction(Q, Unew, dt)
+ * (ThermalConduction.h:32-97) */
+int fv3d_apply_tc(fv3d_ctx* ctx, const fv3d_array* Q, fv3d_array* Unew, double dt);
+/* ViscosityFunctor::applyViscosity(Q, Unew, dt) (Viscosity.h:22-155) */
+int fv3d_apply_visc(fv3d_ctx* ctx, const fv3d_array* Q, fv3d_array* Unew, double dt);
+/* UpdateFunctor::euler_step(Q, Unew, dt) (Update.h:164-178): ghost fill of
+ * Q, slopes (PLM), flux update, then conduction and viscosity if active;
+ * the slopes live in the context, as UpdateFunctor owns its slopesX/Y/Z */
+int fv3d_euler_step(fv3d_ctx* ctx, fv3d_array* Q, fv3d_array* Unew, double dt);
+/* UpdateFunctor::update(Q, Unew, dt) (Update.h:180-206): Euler, or SSP-RK2
+ * with U0 and Ustar of the full (Ntz, Nty, Ntx, 5) shape (the reference
+ * gives them three extents, which aborts in deep_copy); mutates Q as the
+ * reference does.  U0 and Ustar are kept by the context after first use. */
+int fv3d_update(fv3d_ctx* ctx, fv3d_array* Q, fv3d_array* Unew, double dt);
+/* ComputeDtFunctor::computeDt(Q, ...) (ComputeDt.h:17-63) on a primitive
+ * array; synchronises.  FV3D_ENONFINITE as for fv3d_compute_dt. */
+int fv3d_compute_dt_array(fv3d_ctx* ctx, const fv3d_array* Q, double inv[3], double* dt);
+/* the fused path's state <-> the domain of a conserved array, on the
+ * device (ghosts of U untouched); set resets the step count and dt as
+ * fv3d_set_cons does */
+int fv3d_get_cons_array(fv3d_ctx* ctx, fv3d_array* U);
+int fv3d_set_cons_array(fv3d_ctx* ctx, const fv3d_array* U);
+
 /* ---- functor-level entry points on host arrays (unit parity) ---- */
 
 /* BoundaryManager::fillBoundaries (BoundaryConditions.h:146-215) on a host
@@ -100,6 +167,7 @@ int fv3d_fill_boundaries(const fv3d_params* p, double* Q);
 /* hll / hllc (RiemannSolvers.h:6,39) on n face-frame state pairs [n][5]. */
 int fv3d_riemann(int solver, double gamma0, long n, const double* qL, const double* qR,
                  double* flux, double* pout);
+
 
 /* Measurement support (no counterpart in the reference): a plain device copy
  * of nbytes (16-byte accesses), run reps times; *ms_per_copy is the mean

@@ -118,6 +118,10 @@ class Oracle:
         """UpdateFunctor::update: mutates Q and Unew in place."""
         lib().orc_update(ctypes.byref(self.p), _p(Q), _p(Unew), dt, _p(self.work))
 
+    def euler_step(self, Q, Unew, dt):
+        """UpdateFunctor::euler_step: fills Q's ghosts and updates Unew in place."""
+        lib().orc_euler_step(ctypes.byref(self.p), _p(Q), _p(Unew), dt, _p(self.work))
+
     def riemann(self, solver, qL, qR):
         qL = np.ascontiguousarray(qL, dtype=np.float64).reshape(-1, 5)
         qR = np.ascontiguousarray(qR, dtype=np.float64).reshape(-1, 5)

@@ -15,6 +15,18 @@ def pytest_configure(config):
     config.addinivalue_line("filterwarnings", "error::fv3d_amd.NonFiniteWarning")
 
 
+@pytest.fixture(params=[0, 3, 5], ids=["nzc_auto", "nzc3", "nzc5"])
+def nzc(request, monkeypatch):
+    """z-chunk length of the stage kernel for contexts made in the test: the
+    library's own choice (one plane per workgroup on small boxes), or short
+    chunks of 3 and 5 planes, so each workgroup marches several planes --
+    carrying z slopes, faces, the temperature ring and the viscous planes
+    from plane to plane -- and the last chunk of most boxes is partial."""
+    if request.param:
+        monkeypatch.setenv("FV3D_NZC", str(request.param))
+    return request.param
+
+
 @pytest.fixture(scope="session")
 def gpu_available():
     import fv3d_amd

@@ -92,6 +92,7 @@ def test_driver_fp32_runs(gpu_available, tmp_path):
     assert np.all(np.isfinite(rho)) and rho.min() > 0.1
 
 
+@pytest.mark.usefixtures("nzc")
 @pytest.mark.gpu
 def test_driver_c91_settings_match_oracle(gpu_available, tmp_path):
     """The reference's settings/C91.ini setup (C91 z boundaries, conduction

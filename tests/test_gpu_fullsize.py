@@ -159,3 +159,49 @@ def test_state_past_32bit_offsets(gpu_available, precision):
     assert dts_s == dts_b
     assert np.array_equal(w_s, w_b)
     assert not np.array_equal(w_s[0], w_s[10])  # the pulse moved into the compared window
+
+
+@pytest.mark.timeout(1200)
+def test_blast_256_full_run_matches_oracle(gpu_available, capsys):
+    """The whole 256^3 blast run to t = 0.1 through the reference's time
+    loop (main.cpp:51-76) in lockstep with the oracle's: every dt and every
+    snapshot (save_freq 0.01, plus the final stale one) bit-equal,
+    NaN-aware.  The run passes through the reference scheme's own negative
+    pressures (from step 91) and NaN cells; the reference's Max reducer
+    leaves those out of dt, and so do both sides here."""
+    p, Q0 = blast256()
+    p = p.replace(save_freq=0.01)
+    o = Oracle(p)
+    U = o.prim_to_cons(Q0)
+    Unew = o.zeros()
+    t, ite, next_save, steps = 0.0, 0, 0.0, 0
+
+    def same(a, b, what):
+        assert np.array_equal(a, b, equal_nan=True), what
+
+    with F.Simulation(p, F.FP64) as sim, pytest.warns(F.NonFiniteWarning):
+        sim.set_prim(Q0)
+        while t + p.epsilon < p.tend:
+            Unew[...] = U
+            save = t + p.epsilon > next_save
+            Q = o.cons_to_prim(U)
+            dt_o, inv_o = o.compute_dt(Q)
+            dt_g, inv_g = sim.compute_dt()
+            assert dt_g == dt_o and inv_g == inv_o, (steps, dt_g, dt_o)
+            if save:
+                same(sim.get_prim(with_ghosts=False), dom(p, Q), f"snapshot ite_{ite} at step {steps}")
+                ite += 1
+                next_save += p.save_freq
+            o.update(Q, Unew, dt_o)
+            U, Unew = Unew, U
+            sim.step()
+            t += dt_o
+            steps += 1
+            if steps % 20 == 0:
+                with capsys.disabled():
+                    print(f"\nblast 256^3 full run: step {steps}, t={t:.5f}, {ite} snapshots equal",
+                          flush=True)
+        # the final snapshot: the stale Q of main.cpp:76, prim(U*) of the last step
+        same(sim.get_stage_prim(with_ghosts=False), dom(p, Q), "final stale snapshot")
+    # saves at t = 0, 0.01, ..., 0.09 inside the loop, then the final one
+    assert 250 <= steps <= 280 and ite == 10, (steps, ite)

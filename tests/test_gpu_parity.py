@@ -115,6 +115,7 @@ def test_sod_c1_full_run(gpu_available):
     assert len(steps) > 100 and len(snaps) == 11
 
 
+@pytest.mark.usefixtures("nzc")
 def test_blast_reflecting_bitwise(gpu_available):
     p = blast_small(32, tend=1.0)
     Q0 = F.problems.blast(p, radius=0.1, rho_in=1.0, p_in=10.0, rho_out=1.0, p_out=0.1)
@@ -135,6 +136,7 @@ def test_blast_negative_pressure_run_goes_on(gpu_available):
     assert np.nanmin(dom(p, snaps[-1][2])[..., 4]) < 0.0
 
 
+@pytest.mark.usefixtures("nzc")
 @pytest.mark.parametrize("recon", [F.PCM, F.PLM])
 @pytest.mark.parametrize("solver", [F.HLL, F.HLLC])
 @pytest.mark.parametrize("ts", [F.TS_EULER, F.TS_RK2])
@@ -146,6 +148,7 @@ def test_periodic_tc_visc(gpu_available, recon, solver, ts):
     _compare_runs(p, smooth_random(p, 1), max_steps=6)
 
 
+@pytest.mark.usefixtures("nzc")
 @pytest.mark.parametrize("shape", [(24, 2, 3), (2, 3, 24), (3, 24, 2)])
 @pytest.mark.parametrize("bc", [F.BC_PERIODIC, F.BC_REFLECTING, F.BC_ABSORBING])
 def test_smallest_extents(gpu_available, shape, bc):
@@ -158,6 +161,7 @@ def test_smallest_extents(gpu_available, shape, bc):
     _compare_runs(p, smooth_random(p, 5), max_steps=5)
 
 
+@pytest.mark.usefixtures("nzc")
 def test_hll_fixed_temperature_visc_gravity(gpu_available):
     """Partial tiles (16 < 64 columns), HLL, conduction with fixed-temperature
     walls, viscosity and gravity: the configuration family of the
@@ -173,6 +177,7 @@ def test_hll_fixed_temperature_visc_gravity(gpu_available):
     _compare_runs(p, Q0, max_steps=8)
 
 
+@pytest.mark.usefixtures("nzc")
 def test_fixed_gradient_and_absorbing(gpu_available):
     p = Params.defaults().replace(Nx=20, Ny=12, Nz=10, boundary_x=F.BC_ABSORBING, boundary_y=F.BC_REFLECTING,
                                   boundary_z=F.BC_ABSORBING, reconstruction=F.PLM, time_stepping=F.TS_RK2,
@@ -182,6 +187,7 @@ def test_fixed_gradient_and_absorbing(gpu_available):
     _compare_runs(p, smooth_random(p, 3), max_steps=6)
 
 
+@pytest.mark.usefixtures("nzc")
 def test_pcm_wb_well_balanced_gravity(gpu_available):
     p = Params.defaults().replace(Nx=8, Ny=8, Nz=24, boundary_x=F.BC_PERIODIC, boundary_y=F.BC_PERIODIC,
                                   boundary_z=F.BC_REFLECTING, reconstruction=F.PCM_WB,
@@ -190,6 +196,7 @@ def test_pcm_wb_well_balanced_gravity(gpu_available):
     _compare_runs(p, smooth_random(p, 4, amp_u=0.05), max_steps=6)
 
 
+@pytest.mark.usefixtures("nzc")
 def test_partial_tiles_non_pow2(gpu_available):
     """Sizes that fill no tile, and cell sizes that are not powers of two
     (true divisions in the update)."""
@@ -228,6 +235,7 @@ def c91_params(**kw):
     return Params.defaults().replace(**base)
 
 
+@pytest.mark.usefixtures("nzc")
 @pytest.mark.parametrize("ts", [F.TS_EULER, F.TS_RK2])
 def test_c91_z_boundaries_bitwise(gpu_available, ts):
     """boundaries_z=C91 in the fused stage (ghost planes from fillC91,
@@ -237,6 +245,7 @@ def test_c91_z_boundaries_bitwise(gpu_available, ts):
     _compare_runs(p, F.problems.c91(p), max_steps=8)
 
 
+@pytest.mark.usefixtures("nzc")
 def test_c91_z_with_reflecting_x_hll_pcm(gpu_available):
     """C91 z ghosts of reflected x/y ghost columns (edges and corners), HLL,
     PCM, no diffusion."""
@@ -246,6 +255,7 @@ def test_c91_z_with_reflecting_x_hll_pcm(gpu_available):
     _compare_runs(p, F.problems.c91(p), max_steps=8)
 
 
+@pytest.mark.usefixtures("nzc")
 @pytest.mark.parametrize("walls", ["fixed_temperature", "fixed_gradient"])
 def test_b02_conductivity_bitwise(gpu_available, walls):
     """conductivity_mode=B02: the tanh profile of computeKappa
