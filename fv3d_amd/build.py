@@ -31,7 +31,8 @@ HIP_FLAGS = [
 
 SOURCES = ["fv3d_stage.hip", "fv3d_stage_f32.hip", "fv3d_batch.hip", "fv3d_batch_f32.hip",
            "fv3d_aux.hip", "fv3d_api.hip", "fv3d_batch_api.hip", "fv3d_adjoint.hip",
-           "fv3d_adjoint_f32.hip", "fv3d_adjoint_api.hip", "fv3d_functors.hip"]
+           "fv3d_adjoint_f32.hip", "fv3d_adjoint_api.hip", "fv3d_functors.hip",
+           "fv3d_noalloc.hip"]
 # per-source extra flags.  The FP32 kernels flush FP32 denormals (FP64 keeps
 # them): state values sit far above 1e-38, and without denormal support the
 # FP32 square root and division need no range scaling.  They also contract

@@ -168,6 +168,18 @@ int fv3d_fill_boundaries(const fv3d_params* p, double* Q);
 int fv3d_riemann(int solver, double gamma0, long n, const double* qL, const double* qR,
                  double* flux, double* pout);
 
+/* fv3d_riemann without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight host loops, harnesses that forbid hipMalloc).  The faces
+ * are staged in a fixed device array of FV3D_NOALLOC_MAX_FACES faces
+ * (8 MiB) that each device holds until the process ends; n beyond that is
+ * rejected with FV3D_EINVAL.  Runs on the current device's null stream and
+ * returns when the results are in flux and pout.  Calls that overlap on one
+ * device share that array and race. */
+#define FV3D_NOALLOC_MAX_FACES 65536
+int fv3d_riemann_noalloc(int solver, double gamma0, long n, const double* qL, const double* qR,
+                         double* flux, double* pout);
+
 
 /* Measurement support (no counterpart in the reference): a plain device copy
  * of nbytes (16-byte accesses), run reps times; *ms_per_copy is the mean
