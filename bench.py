@@ -80,9 +80,28 @@ def read_valu_per_cell():
     return float(m.group(1)) if m else None
 
 
+def cpu_model():
+    """The host CPU's model name and logical CPU count (/proc/cpuinfo)."""
+    name, n = "unknown", 0
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name") and name == "unknown":
+                    name = line.split(":", 1)[1].strip()
+                if line.startswith("processor"):
+                    n += 1
+    except OSError:
+        pass
+    return name, n
+
+
 def cpu_baseline(n, budget_s):
     """The oracle (CPU restatement, reference pass structure) on the same
-    grid: one warm-up step, then whole steps until budget_s has elapsed."""
+    grid: one warm-up step, then whole steps until budget_s has elapsed.
+    OpenMP threads are bound one per core, packed (OMP_PLACES=cores,
+    OMP_PROC_BIND=close), before the oracle's OpenMP runtime starts."""
+    os.environ.setdefault("OMP_PLACES", "cores")
+    os.environ.setdefault("OMP_PROC_BIND", "close")
     from oracle.oracle import Oracle, lib as olib
     import ctypes
     p = blast_params(n)
@@ -105,11 +124,15 @@ def cpu_baseline(n, budget_s):
         if el >= budget_s or steps >= 20:
             break
     cells = p.Nx * p.Ny * p.Nz
+    model, logical = cpu_model()
     return {
         "value": cells * steps / el / 1e9,
         "unit": "Gcell-updates/s",
         "cores": int(L.orc_num_threads()),
         "kind": "port",
+        "cpu": model,
+        "logical_cpus_visible": logical,
+        "omp": {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PLACES", "OMP_PROC_BIND")},
         "sample": f"blast {n}^3 FP64 RK2, {steps} full steps after 1 warm-up step "
                   f"({el:.1f} s), reference pass structure, OpenMP",
     }
@@ -154,14 +177,60 @@ def measure(F, p, Q0, precision, steps, warmup):
     return total_ms, stage_ms, wall
 
 
-def form_line(F, p, Q0, precision, steps, warmup, alg_bytes):
+def per_stage(stage_ms, cells, stage_bytes):
+    """Roofline of each RK2 stage launch on its own: stage_ms alternates
+    stage 1, stage 2; stage_bytes are their algorithmic bytes per cell."""
+    out = {}
+    for i, (name, b) in enumerate(zip(("stage1", "stage2"), stage_bytes)):
+        ms = float(np.mean(stage_ms[i::2]))
+        gbs = b * cells / (ms * 1e-3) / 1e9
+        out[name] = {"alg_bytes_per_cell": b, "mean_launch_ms": round(ms, 5), "achieved_GBs": round(gbs, 1),
+                     "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    return out
+
+
+# algorithmic bytes per cell of stage 1 and stage 2 (R U^n + W U*; R U*, U^n
+# (+ compensation) + W U^{n+1} (+ compensation))
+STAGE_BYTES = {"f64": (80.0, 120.0), "fp32_kahan": (40.0, 100.0), "fp32_plain": (40.0, 60.0)}
+
+
+def form_line(F, p, Q0, precision, steps, warmup, alg_bytes, stage_bytes):
     cells = p.Nx * p.Ny * p.Nz
     total_ms, stage_ms, _ = measure(F, p, Q0, precision, steps, warmup)
     value = cells * steps / (total_ms * 1e-3) / 1e9
     achieved = alg_bytes * cells / 2.0 / (float(np.mean(stage_ms)) * 1e-3) / 1e9
     return {"value": round(value, 4), "unit": "Gcell-updates/s", "ms_per_step": round(total_ms / steps, 4),
             "alg_bytes_per_cell_update": alg_bytes, "roofline_achieved_GBs": round(achieved, 1),
-            "roofline_frac": round(achieved / HBM_PEAK_GBS, 4)}
+            "roofline_frac": round(achieved / HBM_PEAK_GBS, 4),
+            "per_stage": per_stage(stage_ms, cells, stage_bytes)}
+
+
+def fp32_error_vs_fp64(F, p, Q0, steps):
+    """Per-field error of the FP32 forms against FP64 on the same workload:
+    the FP64 run's dt sequence drives every run (fv3d_step_dt), so all
+    three states are at the same time; L1 and max norms relative to the
+    FP64 field's mean absolute value (velocities: to the FP64 speed's)."""
+    with F.Simulation(p, F.FP64) as sim:
+        sim.set_prim(Q0)
+        dts = []
+        for _ in range(steps):
+            dts.append(sim.compute_dt(strict=True)[0])
+            sim.step()
+        ref = sim.get_prim(with_ghosts=False)
+    speed = float(np.mean(np.sqrt((ref[..., 1:4] ** 2).sum(-1))))
+    scale = [float(np.mean(np.abs(ref[..., 0]))), speed, speed, speed, float(np.mean(np.abs(ref[..., 4])))]
+    out = {"steps": steps, "dt": "FP64 run's sequence for every run",
+           "norm": "|FP32 - FP64| / mean|FP64 field| (u, v, w: / mean FP64 speed)"}
+    for name, prec in (("fp32_kahan_rk2", F.FP32), ("fp32_plain_rk2", F.FP32_PLAIN)):
+        with F.Simulation(p, prec) as sim:
+            sim.set_prim(Q0)
+            for dt in dts:
+                sim.step_dt(dt)
+            q = sim.get_prim(with_ghosts=False)
+        d = np.abs(q - ref)
+        out[name] = {f: {"l1": float(np.mean(d[..., i]) / scale[i]), "linf": float(np.max(d[..., i]) / scale[i])}
+                     for i, f in enumerate(["rho", "u", "v", "w", "prs"])}
+    return out
 
 
 def ensemble_form(F, nbox, steps, tune_cache=None):
@@ -216,7 +285,7 @@ def ensemble_form(F, nbox, steps, tune_cache=None):
             "l1_rho_fp32_vs_fp64_max": max(l1), "l1_rho_fp32_vs_fp64_mean": float(np.mean(l1))}
 
 
-def stratified_form(F, steps=2000):
+def stratified_form(F, steps=2000, dev_window=200):
     """Stratified convection with conduction and viscosity: 256x256x128 on
     [0,2]x[0,2]x[0,1] (dx = 1/128), polytrope m=1 under g=1 (T = 1 + z/2,
     rho = T, p = T^2), gamma 5/3, kappa 0.07, mu 0.005, velocity noise 1e-3,
@@ -249,9 +318,10 @@ def stratified_form(F, steps=2000):
             sim.synchronize()
             wall = time.perf_counter() - t0
             res[name] = (sim.get_prim(with_ghosts=False), trip)
-            tot, _ = sim.time_steps(20)  # device-only rate on the final state
+            tot, _ = sim.time_steps(dev_window)  # device-only rate, continuing from the final state
         out[name] = {"Gcells_incl_dt_readback": round(cells * steps / wall / 1e9, 4),
-                     "Gcells_device": round(cells * 20 / (tot * 1e-3) / 1e9, 4),
+                     "Gcells_device": round(cells * dev_window / (tot * 1e-3) / 1e9, 4),
+                     "device_window_steps": dev_window,
                      "dt_hyp_first_last": [trip[0, 0], trip[-1, 0]], "dt_TC": trip[0, 1], "dt_visc": trip[0, 2]}
         print(f"stratified: {name} done ({wall:.1f} s)", file=sys.stderr, flush=True)
     q64, t64 = res["fp64"]
@@ -309,7 +379,10 @@ def adjoint_form(F, n=128, steps=200, K=20, ndirs=16, precision=None, fd=True, r
         r = s2.get_prim(with_ghosts=False)[..., 0]
         if s2 is not sim:
             s2.close()
-        return float(np.sum((r - target) ** 2)), r, ms
+        # summed in extended precision: the float64 sum's rounding (about
+        # 1e-15 J over 2M cells) would otherwise dominate the central
+        # differences at eps = 1e-6
+        return float(np.sum((r - target) ** 2, dtype=np.longdouble)), r, ms
 
     J0, rT, _ = J_of(d0[..., 0])
     fwd_ms = []
@@ -348,21 +421,40 @@ def adjoint_form(F, n=128, steps=200, K=20, ndirs=16, precision=None, fd=True, r
             "dJ_dkappa_rel": abs(dk - ref["dk"]) / abs(ref["dk"]),
             "dJ_dmu_rel": abs(dm - ref["dm"]) / abs(ref["dm"])}
     if fd and ndirs > 0:
+        # Central differences at three step sizes per direction.  How the
+        # error scales with eps tells its source: a smooth truncation error
+        # falls as eps^2, crossings of the scheme's switch points (minmod
+        # selections, Riemann wave regions: kinks of J along the line) make
+        # an error proportional to eps (the number of kinks crossed grows
+        # with the interval), and rounding grows as 1/eps.  order > 0 from
+        # 1e-5 to 1e-6 means the error falls with eps.
         rng = np.random.default_rng(7)
-        eps = 1e-6
-        rel = []
+        eps_list = (1e-5, 1e-6, 1e-7)
+        rows = []
         for i in range(ndirs):
             # perturbations of the same class as the density field (P(k) ~ k^-2), unit norm
             d = problems.spectral_field(rng, g.shape, -2.0)
             d /= np.linalg.norm(d)
-            Jp = J_of(d0[..., 0] + eps * d)[0]
-            Jm = J_of(d0[..., 0] - eps * d)[0]
-            fdv = (Jp - Jm) / (2 * eps)
             ad = float(np.sum(g * d))
-            rel.append(abs(fdv - ad) / abs(ad))
-        print(f"adjoint: FD rho0 rel errors max {max(rel):.2e}", file=sys.stderr, flush=True)
-        out["fd_rho0_rel_err"] = {"eps": eps, "directions": ndirs, "max": max(rel),
-                                  "median": float(np.median(rel))}
+            errs = []
+            for eps in eps_list:
+                Jp = J_of(d0[..., 0] + eps * d)[0]
+                Jm = J_of(d0[..., 0] - eps * d)[0]
+                errs.append(abs((Jp - Jm) / (2 * eps) - ad) / abs(ad))
+            # the error's order in eps between 1e-5 and 1e-6
+            order = float(np.log10(max(errs[0], 1e-300) / max(errs[1], 1e-300)))
+            rows.append({"adjoint": ad, "rel_err": dict(zip(["1e-5", "1e-6", "1e-7"], errs)),
+                         "order_1e5_to_1e6": round(order, 2)})
+        rel = [r["rel_err"]["1e-6"] for r in rows]
+        best = [min(r["rel_err"].values()) for r in rows]
+        print(f"adjoint: FD rho0 rel errors at eps 1e-6: max {max(rel):.2e}; best over eps: max "
+              f"{max(best):.2e}", file=sys.stderr, flush=True)
+        out["fd_rho0_rel_err"] = {
+            "eps": 1e-6, "directions": ndirs, "max": max(rel), "median": float(np.median(rel)),
+            "over_1e-6": int(sum(r > 1e-6 for r in rel)),
+            "best_eps_max": max(best),
+            "per_direction": rows,
+            "order_note": "order ~2: smooth truncation; ~1: switch-point crossings; ~-1: rounding"}
         for key, val, ad in [("kappa", p.kappa, dk), ("mu", p.mu, dm)]:
             h = eps * val
             Jp = J_of(d0[..., 0], p.replace(**{key: val + h}))[0]
@@ -386,6 +478,8 @@ def main():
     ap.add_argument("--ensemble", type=int, default=512, help="boxes of the batched ensemble (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--stratified", type=int, default=2000, help="steps of the stratified-convection form (0: skip)")
+    ap.add_argument("--stratified-window", type=int, default=200,
+                    help="steps of its device-only timing window")
     ap.add_argument("--adjoint", type=int, default=128, help="grid of the adjoint workload (0: skip)")
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--adjoint-dirs", type=int, default=16)
@@ -444,6 +538,7 @@ def main():
             "kernel": "fv3d_hip::stage_kernel<double,8,PLM,HLLC,false,false,STAGE_PLAIN,STAGE_VAR_LEAN>",
             "alg_bytes_per_launch": bytes_per_launch,
             "mean_launch_ms": round(mean_launch_ms, 5),
+            "per_stage": per_stage(stage_ms, cells, STAGE_BYTES["f64"]),
         },
     }
     valu = read_valu_per_cell()
@@ -458,13 +553,17 @@ def main():
     if args.forms:
         # the FP32 forms on the same workload (error against FP64 in tests/)
         out["forms"] = {
-            "fp32_kahan_rk2": form_line(F, p, Q0, F.FP32, args.steps, args.warmup, 140.0),
-            "fp32_plain_rk2": form_line(F, p, Q0, F.FP32_PLAIN, args.steps, args.warmup, 100.0),
+            "fp32_kahan_rk2": form_line(F, p, Q0, F.FP32, args.steps, args.warmup, 140.0,
+                                        STAGE_BYTES["fp32_kahan"]),
+            "fp32_plain_rk2": form_line(F, p, Q0, F.FP32_PLAIN, args.steps, args.warmup, 100.0,
+                                        STAGE_BYTES["fp32_plain"]),
+            "fp32_error_vs_fp64": fp32_error_vs_fp64(F, p, Q0, min(60, args.steps + args.warmup)),
         }
     if args.ensemble > 0:
         out.setdefault("forms", {})["fp32_ensemble_rt"] = ensemble_form(F, args.ensemble, args.ensemble_steps)
     if args.stratified > 0:
-        out.setdefault("forms", {})["stratified_tc_visc"] = stratified_form(F, args.stratified)
+        out.setdefault("forms", {})["stratified_tc_visc"] = stratified_form(F, args.stratified,
+                                                                             args.stratified_window)
     if args.adjoint > 0:
         a64 = adjoint_form(F, args.adjoint, args.adjoint_steps, ndirs=args.adjoint_dirs)
         a32 = adjoint_form(F, args.adjoint, args.adjoint_steps, precision=F.FP32_PLAIN, fd=False,

@@ -36,7 +36,7 @@ VSC_CONSTANT = 0
 
 FP64, FP32, FP32_PLAIN = 0, 1, 2
 
-ERRORS = {1: "EINVAL", 2: "EHIP", 3: "ENOMEM", 4: "ENONFINITE"}
+ERRORS = {1: "EINVAL", 2: "EHIP", 3: "ENOMEM", 4: "ENONFINITE", 5: "EFLOOR"}
 ENONFINITE = 4
 
 

@@ -75,6 +75,7 @@ def build_library(verbose: bool = False) -> str:
 
 
 DRIVER = os.path.join(HERE, "fv3d")
+FUNCTOR_CHECK = os.path.join(HERE, "fv3d_functor_check")
 DRIVER_SOURCES = ["main.cpp", "sim_info.cpp", "init.cpp", "io_manager.cpp"]
 HDF5_ROOT = os.environ.get("FV3D_HDF5_ROOT", "/opt/conda")
 
@@ -100,8 +101,14 @@ def build_driver(verbose: bool = False) -> str:
            "-o", tmp, LIB, hdf5_lib, "-Wl,-rpath,$ORIGIN:$ORIGIN/_hdf5"]
     _run(cmd)
     os.replace(tmp, DRIVER)
+    # the functor facades' check program (no HDF5)
+    tmp = FUNCTOR_CHECK + ".tmp"
+    _run([os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", f"-I{INCLUDE}",
+          f"-I{os.path.join(HERE, 'driver')}", os.path.join(HERE, "driver", "functor_check.cpp"),
+          "-o", tmp, LIB, "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, FUNCTOR_CHECK)
     if verbose:
-        print("built", DRIVER)
+        print("built", DRIVER, "and", FUNCTOR_CHECK)
     return DRIVER
 
 

@@ -58,21 +58,51 @@ StageLaunch<real> frozen_stage(const fv3d_ctx* c, const double* dt_slot) {
   return L;
 }
 
+// The density floor fmax(1e-6, rho) (Update.h:160) has acted on a stage
+// output exactly where its density is the floor value (the floor maps every
+// smaller or NaN density there; a density that lands on 1e-6 by itself is a
+// kink of the map as well).  Inside an adjoint run, which differentiates the
+// map without the floor, every frozen stage output is scanned and *flag set
+// if the floor acted.
+template <typename real>
+__global__ void __launch_bounds__(256) k_floor_scan(const real* __restrict__ rho, long long n,
+                                                    int* flag) {
+  int hit = 0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    hit |= rho[i] == (real)1.0e-6;
+  if (__any(hit) && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
+}
+
+template <typename real>
+hipError_t floor_scan(const fv3d_ctx* c, const real* U) {
+  if (!c->adj_floor) return hipSuccess;
+  const long long n = (long long)c->p.Nx * c->p.Ny * c->p.Nz;
+  const long long b = std::min<long long>((n + 255) / 256, 2048);
+  hipLaunchKernelGGL((k_floor_scan<real>), dim3((unsigned)b), dim3(256), 0, c->stream, U, n,
+                     c->adj_floor);
+  return hipGetLastError();
+}
+
 // one frozen-dt step: U0 -> U1 (stage state, RK2 only) -> Unext.  Unext may
 // alias U0.  For Euler, U1 is unused and Unext must differ from U0.
 template <typename real>
 hipError_t frozen_step(const fv3d_ctx* c, const double* dt_slot, const real* U0, real* U1,
                        real* Unext) {
   StageLaunch<real> L = frozen_stage<real>(c, dt_slot);
+  hipError_t e;
   if (c->p.time_stepping == FV3D_TS_RK2) {
     L.Uin = U0; L.Uout = U1;
-    hipError_t e = launch_stage<real>(L, c->stream);
+    e = launch_stage<real>(L, c->stream);
+    if (e == hipSuccess) e = floor_scan<real>(c, U1);
     if (e != hipSuccess) return e;
     L.Uin = U1; L.Urk = U0; L.Uout = Unext;
-    return launch_stage<real>(L, c->stream);
+  } else {
+    L.Uin = U0; L.Uout = Unext;
   }
-  L.Uin = U0; L.Uout = Unext;
-  return launch_stage<real>(L, c->stream);
+  e = launch_stage<real>(L, c->stream);
+  if (e == hipSuccess) e = floor_scan<real>(c, Unext);
+  return e;
 }
 
 template <typename real>
@@ -124,6 +154,18 @@ int adjoint_run_t(fv3d_ctx* c, int nsteps, int K, const double* dts_dev,
   const size_t ngrad = 2 + 2 * (size_t)N;
   HIPCHK(c, hipMalloc(&grad, ngrad * sizeof(double)));
   struct GradFree { double* p; ~GradFree() { hipFree(p); } } gf{grad};
+  // every frozen stage of this run (forward pass and recomputation) is
+  // scanned for densities the floor fmax(1e-6, rho) (Update.h:160) set:
+  // the adjoint differentiates the map without it
+  int* floor_flag = nullptr;
+  HIPCHK(c, hipMalloc(&floor_flag, sizeof(int)));
+  struct FloorGuard {
+    fv3d_ctx* c;
+    int* p;
+    ~FloorGuard() { c->adj_floor = nullptr; hipFree(p); }
+  } fg{c, floor_flag};
+  HIPCHK(c, hipMemsetAsync(floor_flag, 0, sizeof(int), c->stream));
+  c->adj_floor = floor_flag;
   HIPCHK(c, hipMemsetAsync(grad, 0, ngrad * sizeof(double), c->stream));
   HIPCHK(c, hipMemsetAsync(Qb, 0, 6 * (size_t)N * sizeof(real), c->stream));
 
@@ -207,11 +249,18 @@ int adjoint_run_t(fv3d_ctx* c, int nsteps, int K, const double* dts_dev,
         HIPCHK(c, frozen_step<real>(c, dts_dev + s, S0 + (size_t)i * ns,
                                     rk2 ? S1 + (size_t)i * ns : nullptr, S0 + (size_t)(i + 1) * ns));
       }
-      if (rk2) {  // the last step's stage state (its output is not needed)
+      real* last0 = S0 + (size_t)(s1 - 1 - s0) * ns;
+      real* last1 = rk2 ? S1 + (size_t)(s1 - 1 - s0) * ns : nullptr;
+      if (cp == ncp - 1) {
+        // the run's final step in full: its output U^n is scanned for the
+        // density floor like every other stage output (tmp is free now)
+        HIPCHK(c, frozen_step<real>(c, dts_dev + (s1 - 1), last0, last1, tmp));
+      } else if (rk2) {  // the last step's stage state (its output is not needed)
         StageLaunch<real> L = frozen_stage<real>(c, dts_dev + (s1 - 1));
-        L.Uin = S0 + (size_t)(s1 - 1 - s0) * ns;
-        L.Uout = S1 + (size_t)(s1 - 1 - s0) * ns;
+        L.Uin = last0;
+        L.Uout = last1;
         HIPCHK(c, launch_stage<real>(L, c->stream));
+        HIPCHK(c, floor_scan<real>(c, last1));
       }
     }
     for (int s = s1 - 1; s >= s0; --s) {
@@ -244,9 +293,15 @@ int adjoint_run_t(fv3d_ctx* c, int nsteps, int K, const double* dts_dev,
   c->adj_ms[2] = fwd + bwd;
 
   std::vector<real> h(ns);
+  int floor_hit = 0;
   HIPCHK(c, hipMemcpyAsync(h.data(), lam, ns * sizeof(real), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemcpyAsync(grads, grad, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(&floor_hit, floor_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (floor_hit)
+    return ctx_fail(c, FV3D_EFLOOR,
+                    "adjoint: the density floor fmax(1e-6, rho) (Update.h:160) changed a density in "
+                    "this run; the adjoint takes it as inactive, so the gradient would be wrong");
   lam0.assign(h.begin(), h.end());
   return FV3D_OK;
 }

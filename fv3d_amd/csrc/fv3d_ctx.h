@@ -34,6 +34,7 @@ struct fv3d_ctx {
   void* fn_arrays[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   unsigned long long* fn_acc = nullptr;
   float adj_ms[3] = {0, 0, 0};  // last adjoint run: forward pass, backward pass, total
+  int* adj_floor = nullptr;     // during an adjoint run: its frozen stages' density-floor flag
   size_t adj_store_bytes = 0;   // device memory for segments kept from the forward pass
   std::string err;
 };

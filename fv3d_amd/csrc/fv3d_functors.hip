@@ -658,3 +658,61 @@ int fv3d_set_cons_array(fv3d_ctx* c, const fv3d_array* U) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// state-level primToCons / consToPrim (States.h:20-45) on [n][5] states
+// ---------------------------------------------------------------------------
+namespace fv3d_hip {
+__global__ void k_states_convert(int to_cons, long n, double gm1, const double* __restrict__ in,
+                                 double* __restrict__ out) {
+  const long s = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const double* a = in + 5 * s;
+  double* b = out + 5 * s;
+  if (to_cons) {
+    prim_to_cons(Prim<double>{a[0], a[1], a[2], a[3], a[4]}, gm1, b);
+  } else {
+    const Prim<double> q = cons_to_prim<double>(a[0], a[1], a[2], a[3], a[4], gm1);
+    b[0] = q.r; b[1] = q.u; b[2] = q.v; b[3] = q.w; b[4] = q.p;
+  }
+}
+}  // namespace fv3d_hip
+
+namespace {
+int convert_states(int to_cons, double gamma0, long n, const double* in, double* out) {
+  if (n < 0 || (n > 0 && (!in || !out))) return record_error(FV3D_EINVAL, "bad argument");
+  if (n == 0) return FV3D_OK;
+  double *din = nullptr, *dout = nullptr;
+  hipStream_t s;
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess)
+    return record_error(FV3D_EHIP, "hipStreamCreate failed");
+  const size_t b5 = (size_t)n * 5 * sizeof(double);
+  hipError_t e = hipMalloc(&din, b5);
+  if (e == hipSuccess) e = hipMalloc(&dout, b5);
+  if (e == hipSuccess) e = hipMemcpyAsync(din, in, b5, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_states_convert, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                       to_cons, n, gamma0 - 1.0, din, dout);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(out, dout, b5, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (din) hipFree(din);
+  if (dout) hipFree(dout);
+  hipStreamDestroy(s);
+  if (e != hipSuccess) return record_error(FV3D_EHIP, std::string("states: ") + hipGetErrorString(e));
+  return FV3D_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int fv3d_prim_to_cons_states(double gamma0, long n, const double* q, double* u) {
+  return convert_states(1, gamma0, n, q, u);
+}
+
+int fv3d_cons_to_prim_states(double gamma0, long n, const double* u, double* q) {
+  return convert_states(0, gamma0, n, u, q);
+}
+
+}  // extern "C"

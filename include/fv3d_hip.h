@@ -30,6 +30,7 @@ extern "C" {
 #define FV3D_EHIP 2        /* a HIP runtime call failed */
 #define FV3D_ENOMEM 3      /* device allocation failed */
 #define FV3D_ENONFINITE 4  /* the state produced NaN/inf (pressure < 0, density 0, ...) */
+#define FV3D_EFLOOR 5      /* fv3d_adjoint_run: the density floor acted in the run it differentiates */
 
 #define FV3D_FP64 0        /* bit-equal to the reference's FP64 arithmetic */
 #define FV3D_FP32 1        /* FP32 state and fluxes; RK2 accumulation Kahan-compensated */
@@ -164,6 +165,10 @@ int fv3d_set_cons_array(fv3d_ctx* ctx, const fv3d_array* U);
 /* BoundaryManager::fillBoundaries (BoundaryConditions.h:146-215) on a host
  * primitive array with ghosts, run on the device. */
 int fv3d_fill_boundaries(const fv3d_params* p, double* Q);
+/* State primToCons / consToPrim (States.h:20-45) of n states [n][5] with
+ * the given gamma0, run on the device. */
+int fv3d_prim_to_cons_states(double gamma0, long n, const double* q, double* u);
+int fv3d_cons_to_prim_states(double gamma0, long n, const double* u, double* q);
 /* hll / hllc (RiemannSolvers.h:6,39) on n face-frame state pairs [n][5]. */
 int fv3d_riemann(int solver, double gamma0, long n, const double* qL, const double* qR,
                  double* flux, double* pout);
@@ -179,7 +184,6 @@ int fv3d_riemann(int solver, double gamma0, long n, const double* qL, const doub
 #define FV3D_NOALLOC_MAX_FACES 65536
 int fv3d_riemann_noalloc(int solver, double gamma0, long n, const double* qL, const double* qR,
                          double* flux, double* pout);
-
 
 /* Measurement support (no counterpart in the reference): a plain device copy
  * of nbytes (16-byte accesses), run reps times; *ms_per_copy is the mean
@@ -204,7 +208,9 @@ int fv3d_stream_copy(size_t nbytes, int reps, float* ms_per_copy);
  * dJ/dmu (constant conductivity and viscosity).  The forward pass keeps a
  * checkpoint every K steps; each K-step segment is recomputed in the
  * backward pass.  Branches (minmod selection, Riemann wave region) are
- * those of the forward evaluation; the density floor is taken as inactive.
+ * those of the forward evaluation; the density floor is taken as inactive,
+ * and a run in which it changes a density fails with FV3D_EFLOOR (no
+ * gradient is written).
  * Gravity and the well-balanced variants are rejected (FV3D_EINVAL).  The
  * context's state is unchanged.
  * fv3d_adjoint_times gives the device times of the last fv3d_adjoint_run:
