@@ -68,7 +68,7 @@ int fv3d_create(const fv3d_params* p, int precision, int device, fv3d_ctx** out)
   c->p = *p;
   c->precision = precision;
   c->device = device;
-  c->dp = make_devparams(*p, 0, device_cus(device));
+  c->dp = make_devparams(*p, 0, device_cus(device), precision == FV3D_FP64);
   c->elem = precision == FV3D_FP64 ? sizeof(double) : sizeof(float);
   c->state_bytes = (size_t)5 * p->Nx * p->Ny * p->Nz * c->elem;
   auto cleanup = [&](int code, const std::string& m) {

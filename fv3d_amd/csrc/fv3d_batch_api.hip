@@ -86,8 +86,8 @@ static int build_worklist(fv3d_batch* b) {
     const fv3d_params& p = b->params[i];
     int nzc = b->nzc_cap;
     if (nzc <= 0) nzc = std::min(p.Nz, 32);
-    b->dp[i] = make_devparams(p, nzc, b->num_cus);
-    b->dp_loop[i] = make_devparams(p, 0, b->num_cus);
+    b->dp[i] = make_devparams(p, nzc, b->num_cus, b->precision == FV3D_FP64);
+    b->dp_loop[i] = make_devparams(p, 0, b->num_cus, b->precision == FV3D_FP64);
     DevParams& d = b->dp[i];
     desc[i].P = d;
     desc[i].off = b->off[i];

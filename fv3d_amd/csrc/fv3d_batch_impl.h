@@ -105,9 +105,9 @@ static hipError_t batch_launch_t(const BatchLaunch& L, hipStream_t s) {
 
 template <typename real, int RECON, int RS, int MODE, int VAR>
 static hipError_t batch_launch_phys(const BatchLaunch& L, hipStream_t s) {
-  if (L.tc && L.visc) return batch_launch_t<real, STAGE_TY_VISC, RECON, RS, true, true, MODE, VAR>(L, s);
+  if (L.tc && L.visc) return batch_launch_t<real, stage_ty_visc<real>(), RECON, RS, true, true, MODE, VAR>(L, s);
   if (L.tc) return batch_launch_t<real, STAGE_TY, RECON, RS, true, false, MODE, VAR>(L, s);
-  if (L.visc) return batch_launch_t<real, STAGE_TY_VISC, RECON, RS, false, true, MODE, VAR>(L, s);
+  if (L.visc) return batch_launch_t<real, stage_ty_visc<real>(), RECON, RS, false, true, MODE, VAR>(L, s);
   return batch_launch_t<real, STAGE_TY, RECON, RS, false, false, MODE, VAR>(L, s);
 }
 

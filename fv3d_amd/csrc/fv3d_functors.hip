@@ -249,7 +249,7 @@ __global__ void k_arr_viscosity(const double* __restrict__ Q, double* __restrict
       return Q[s + (di - 1) + (dj - 1) * g.sy + (dk - 1) * g.sz + v * g.sf];
     };
     double vx[5] = {0, 0, 0, 0, 0}, vy[5] = {0, 0, 0, 0, 0}, vz[5] = {0, 0, 0, 0, 0};
-    viscous_fluxes<double>(S, 1.0 / P.dx, 1.0 / P.dy, 1.0 / P.dz, P.mu, vx, vy, vz);
+    viscous_fluxes_by_side<double>(S, 1.0 / P.dx, 1.0 / P.dy, 1.0 / P.dz, P.mu, vx, vy, vz);
     // Viscosity.h:149-151: un += dt * (vf_x + vf_y + vf_z), component 0 included
 #pragma unroll
     for (int n = 0; n < 5; ++n)

@@ -99,8 +99,9 @@ inline int device_cus(int device) {
   return cus;
 }
 
-// kernel parameters of a box; cus sizes the z chunks (tiles_z_for)
-inline DevParams make_devparams(const fv3d_params& p, int nzc_override, int cus) {
+// kernel parameters of a box for FP64 or FP32 state; cus sizes the z
+// chunks (tiles_z_for)
+inline DevParams make_devparams(const fv3d_params& p, int nzc_override, int cus, bool fp64) {
   DevParams d;
   std::memset(&d, 0, sizeof d);
   d.Nx = p.Nx; d.Ny = p.Ny; d.Nz = p.Nz;
@@ -142,7 +143,7 @@ inline DevParams make_devparams(const fv3d_params& p, int nzc_override, int cus)
   d.theta1 = p.theta1;
   d.c91g = nullptr;  // set by the owner of the ghost-plane buffer
   d.tiles_x = (p.Nx + STAGE_WX - 1) / STAGE_WX;
-  const int ty = p.viscosity_active ? STAGE_TY_VISC : STAGE_TY;
+  const int ty = stage_ty(p.viscosity_active != 0, fp64);
   d.tiles_y = (p.Ny + ty - 1) / ty;
   d.nzc = nzc_override > 0 ? std::max(1, std::min(nzc_override, p.Nz))
                            : tiles_z_for(p, d.tiles_x * d.tiles_y, cus);

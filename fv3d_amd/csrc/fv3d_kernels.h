@@ -8,9 +8,18 @@ namespace fv3d_hip {
 
 // rows per stage tile (the tile is 64 x STAGE_TY columns)
 constexpr int STAGE_TY = 8;
-// with viscosity: 4 rows (8 waves), which leaves 256 VGPRs per lane for the
-// viscous stencil instead of spilling at 168
-constexpr int STAGE_TY_VISC = 4;
+// with viscosity: FP32 keeps 8 rows (12 waves, 168 VGPRs, no spills); FP64
+// takes 4 rows (8 waves, 231 of 256 VGPRs, no spills), where 8 rows would
+// spill about 80 VGPRs and run 27% slower on the stratified box
+constexpr int STAGE_TY_VISC_F64 = 4;
+constexpr int STAGE_TY_VISC_F32 = 8;
+template <typename real>
+constexpr int stage_ty_visc() {
+  return sizeof(real) == 8 ? STAGE_TY_VISC_F64 : STAGE_TY_VISC_F32;
+}
+inline int stage_ty(bool visc, bool fp64) {
+  return visc ? (fp64 ? STAGE_TY_VISC_F64 : STAGE_TY_VISC_F32) : STAGE_TY;
+}
 constexpr int STAGE_WX = 64;
 
 // Stage kernel variants.  STAGE_VAR_GENERAL tests the optional features at

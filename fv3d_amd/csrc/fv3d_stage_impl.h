@@ -146,12 +146,21 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
   // while slower waves still read this one; viscosity keeps one buffer (its
   // extra planes leave no LDS for a second) and a barrier instead.
   constexpr int NBUF = VISC ? 1 : 2;
+  // viscosity: FP32 evaluates each x and y face once in the face phase
+  // (VFACE); FP64 evaluates a cell's six sides in the update from one
+  // 3x3x3 stencil, whose shared LDS reads cost less than re-reading each
+  // face's stencil (measured 8% faster on the stratified box)
+  constexpr bool VFACE = VISC && sizeof(real) == 4;
   __shared__ real sqb[NBUF][5][NR][NC];
   __shared__ real sTb[NBUF][TC ? NR : 1][TC ? NC : 1];
   __shared__ real svm[VISC ? 3 : 1][VISC ? NR : 1][VISC ? NC : 1];
   __shared__ real svp[VISC ? 3 : 1][VISC ? NR : 1][VISC ? NC : 1];
   __shared__ real sfx[5][TY][WX + 1];
   __shared__ real sfy[5][TY + 1][WX];
+  // viscosity: mu * (stress terms) of each x face and y face, evaluated once
+  // (a face's two cells read the same values in the same order)
+  __shared__ real svx[VFACE ? 4 : 1][VFACE ? TY : 1][VFACE ? WX + 1 : 1];
+  __shared__ real svy[VFACE ? 4 : 1][VFACE ? TY + 1 : 1][VFACE ? WX : 1];
   __shared__ double sred[NR];
 
   const int lane = threadIdx.x & 63;
@@ -260,6 +269,15 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
     auto lds_prim = [&](int rr, int cc) -> Prim<real> {
       return Prim<real>{sq[0][rr][cc], sq[1][rr][cc], sq[2][rr][cc], sq[3][rr][cc], sq[4][rr][cc]};
     };
+    // Viscosity.h's stencil[dk][dj][di] of u, v, w around window cell (rr, cc)
+    auto vstencil = [&](int rr0, int cc0) {
+      return [&, rr0, cc0](int dk, int dj, int di, int v) -> real {
+        const int rr = rr0 + dj - 1, cc = cc0 + di - 1;
+        if (dk == 1) return sq[v][rr][cc];
+        if (dk == 0) return svm[v - 1][rr][cc];
+        return svp[v - 1][rr][cc];
+      };
+    };
     const Prim<real> P2 = load_plane(k + 2);
     real T2 = 0;
     if (TC) T2 = qdiv(P2.p, P2.r);
@@ -298,7 +316,7 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
     // so their latency hides behind the face solves
     const long long cell = (long long)gi + (long long)gj * P.sy + (long long)k * P.sz;
     real ub[5], ur[5], cc[5];
-    if (store) {
+    auto fetch_base = [&]() {
 #pragma unroll
       for (int n = 0; n < 5; ++n) ub[n] = Uin[cell + n * P.sf];
       if (Urk) {
@@ -309,7 +327,12 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
 #pragma unroll
         for (int n = 0; n < 5; ++n) cc[n] = Cio[cell + n * P.sf];
       }
-    }
+    };
+    // with viscosity the face phase also evaluates the viscous faces, and
+    // the base state is fetched after it, so that it does not hold 10-15
+    // registers across the face solves
+    constexpr bool LATE_BASE = VISC && sizeof(real) == 4;
+    if (store && !LATE_BASE) fetch_base();
 
     // ---- x faces: face between window columns c-1 and c for every interior row
     if (interior) {
@@ -357,10 +380,32 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
                         (real)P.entho, Fzh, pouth);
       unrotate_flux<real, FV3D_IZ>(Fzh);
     }
+    if (VFACE) {
+      // viscous x face c-1/2 of the interior rows (side 1 of cell c) and the
+      // last column's right face; y face r-1/2 of the interior columns
+      if (interior) {
+        real t[5];
+        visc_x_side<real>(vstencil(r, c), 1, oodx, oody, oodz, t);
+#pragma unroll
+        for (int n = 1; n < 5; ++n) svx[n - 1][r - 2][lane] = mu * t[n];
+      } else if (wave == NR - 1 && lane < TY) {
+        real t[5];
+        visc_x_side<real>(vstencil(lane + 2, WX + 2), 1, oodx, oody, oodz, t);
+#pragma unroll
+        for (int n = 1; n < 5; ++n) svx[n - 1][lane][WX] = mu * t[n];
+      }
+      if (yworker) {
+        real t[5];
+        visc_y_side<real>(vstencil(r, c), 1, oodx, oody, oodz, t);
+#pragma unroll
+        for (int n = 1; n < 5; ++n) svy[n - 1][r - 2][lane] = mu * t[n];
+      }
+    }
     __syncthreads();
 
     // ---- update (Update.h:145-160, then ThermalConduction.h, Viscosity.h)
     if (store) {
+      if (LATE_BASE) fetch_base();
       real u[5];
 #pragma unroll
       for (int n = 0; n < 5; ++n) u[n] = ub[n];
@@ -414,15 +459,37 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
                                                  dtdz);
       }
 
-      if (VISC) {  // Viscosity.h:28-153, stencil[dk][dj][di] of u, v, w
+      if (VISC && !VFACE) {  // Viscosity.h:28-153, six sides from one stencil
+        real vx[5] = {0, 0, 0, 0, 0}, vy[5] = {0, 0, 0, 0, 0}, vz[5] = {0, 0, 0, 0, 0};
         auto S = [&](int dk, int dj, int di, int v) -> real {
           const int rr = r + dj - 1, cc = c + di - 1;
           if (dk == 1) return sq[v][rr][cc];
           if (dk == 0) return svm[v - 1][rr][cc];
           return svp[v - 1][rr][cc];
         };
-        real vx[5] = {0, 0, 0, 0, 0}, vy[5] = {0, 0, 0, 0, 0}, vz[5] = {0, 0, 0, 0, 0};
         viscous_fluxes<real>(S, oodx, oody, oodz, mu, vx, vy, vz);
+#pragma unroll
+        for (int n = 1; n < 5; ++n) u[n] = u[n] + (vx[n] + vy[n] + vz[n]) * dt;
+      }
+      if (VFACE) {  // Viscosity.h:28-153
+        // x and y: the two faces' terms from LDS; side 1 adds (-mu) t =
+        // -(mu t) to the sum that starts at 0, side 2 adds mu t
+        real vx[5] = {0, 0, 0, 0, 0}, vy[5] = {0, 0, 0, 0, 0}, vz[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+        for (int n = 1; n < 5; ++n) {
+          vx[n] += -svx[n - 1][r - 2][lane];
+          vx[n] += svx[n - 1][r - 2][lane + 1];
+          vy[n] += -svy[n - 1][r - 2][lane];
+          vy[n] += svy[n - 1][r - 1][lane];
+        }
+        // z: both sides per cell (the reference's z branch differs between
+        // a face's two cells)
+#pragma unroll
+        for (int sd = 1; sd < 3; ++sd) {
+          real t[5];
+          visc_z_side<real>(vstencil(r, c), sd, oodx, oody, oodz, t);
+          visc_accumulate(vz, sd, mu, t);
+        }
 #pragma unroll
         for (int n = 1; n < 5; ++n) u[n] = u[n] + (vx[n] + vy[n] + vz[n]) * dt;
       }
@@ -531,9 +598,9 @@ static hipError_t launch_t(const StageLaunch<real>& L, hipStream_t s) {
 template <typename real, int RECON, int RS, int MODE, int VAR>
 static hipError_t launch_phys(const StageLaunch<real>& L, hipStream_t s) {
   const bool tc = L.tc, vi = L.visc;
-  if (tc && vi) return launch_t<real, STAGE_TY_VISC, RECON, RS, true, true, MODE, VAR>(L, s);
+  if (tc && vi) return launch_t<real, stage_ty_visc<real>(), RECON, RS, true, true, MODE, VAR>(L, s);
   if (tc) return launch_t<real, STAGE_TY, RECON, RS, true, false, MODE, VAR>(L, s);
-  if (vi) return launch_t<real, STAGE_TY_VISC, RECON, RS, false, true, MODE, VAR>(L, s);
+  if (vi) return launch_t<real, stage_ty_visc<real>(), RECON, RS, false, true, MODE, VAR>(L, s);
   return launch_t<real, STAGE_TY, RECON, RS, false, false, MODE, VAR>(L, s);
 }
 
