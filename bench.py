@@ -256,7 +256,8 @@ def ensemble_form(F, nbox, steps, tune_cache=None):
     b32 = fresh(F.FP32)
     cells = b32.cells
     nzc, grid = b32.tune(nsteps=3, cache=tune_cache)
-    print(f"ensemble: tuned nzc={nzc} grid={grid}", file=sys.stderr, flush=True)
+    wx = b32.tile_width
+    print(f"ensemble: tuned tile width={wx} nzc={nzc} grid={grid}", file=sys.stderr, flush=True)
     ms_graph = b32.time(steps, F.BATCH_GRAPH)
     b32.compute_dt()  # raises on non-finite state
     print(f"ensemble: graph {ms_graph:.1f} ms / {steps} steps", file=sys.stderr, flush=True)
@@ -277,7 +278,7 @@ def ensemble_form(F, nbox, steps, tune_cache=None):
     b64.close()
     g = cells * steps / (ms_graph * 1e-3) / 1e9
     lo = cells * steps / (ms_loop * 1e-3) / 1e9
-    return {"boxes": nbox, "cells": cells, "steps": steps, "l1_after_steps": b32_steps, "tuned_nzc": nzc, "tuned_grid": grid,
+    return {"boxes": nbox, "cells": cells, "steps": steps, "l1_after_steps": b32_steps, "tuned_tile_width": wx, "tuned_nzc": nzc, "tuned_grid": grid,
             "graph_Gcells": round(g, 4), "loop_Gcells": round(lo, 4), "speedup_vs_loop": round(g / lo, 3),
             "graph_ms_per_step": round(ms_graph / steps, 4), "loop_ms_per_step": round(ms_loop / steps, 4),
             "alg_bytes_per_cell_update": 140.0,

@@ -207,6 +207,10 @@ def lib() -> ctypes.CDLL:
         "fv3d_batch_step_count": (ctypes.c_long, [vp]),
         "fv3d_batch_cells": (ctypes.c_longlong, [vp]),
         "fv3d_batch_configure": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
+        "fv3d_batch_set_tile_width": (ctypes.c_int, [vp, ctypes.c_int]),
+        "fv3d_batch_tile_width": (ctypes.c_int, [vp]),
+        "fv3d_batch_tune_shape": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_char_p, P(ctypes.c_int),
+                                                 P(ctypes.c_int), P(ctypes.c_int)]),
         "fv3d_batch_tune": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_char_p, P(ctypes.c_int),
                                            P(ctypes.c_int)]),
     }
@@ -565,11 +569,21 @@ class Batch:
     def configure(self, nzc_cap: int = 0, grid: int = 0):
         self._check(lib().fv3d_batch_configure(self._b, int(nzc_cap), int(grid)))
 
+    @property
+    def tile_width(self) -> int:
+        """Stage tile width (64 or 32 columns) of the persistent kernels."""
+        return int(lib().fv3d_batch_tile_width(self._b))
+
+    def set_tile_width(self, wx: int = 0):
+        """64 or 32 columns, or 0 for the default (by tile cover)."""
+        self._check(lib().fv3d_batch_set_tile_width(self._b, int(wx)))
+
     def tune(self, nsteps: int = 3, cache: Optional[str] = None):
-        """Autotune the persistent launch shape; returns (nzc_cap, grid)."""
-        nzc, grid = ctypes.c_int(), ctypes.c_int()
-        self._check(lib().fv3d_batch_tune(self._b, int(nsteps), cache.encode() if cache else None,
-                                          ctypes.byref(nzc), ctypes.byref(grid)))
+        """Autotune the persistent launch shape (tile width, z-chunk cap,
+        grid); returns (nzc_cap, grid); the width is `tile_width` after."""
+        nzc, grid, wx = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self._check(lib().fv3d_batch_tune_shape(self._b, int(nsteps), cache.encode() if cache else None,
+                                                ctypes.byref(nzc), ctypes.byref(grid), ctypes.byref(wx)))
         return nzc.value, grid.value
 
 

@@ -29,8 +29,8 @@ HIP_FLAGS = [
     "-Wno-unused-value", "-Wno-unused-result", f"-I{INCLUDE}", f"-I{CSRC}",
 ]
 
-SOURCES = ["fv3d_stage.hip", "fv3d_stage_f32.hip", "fv3d_batch.hip", "fv3d_batch_f32.hip",
-           "fv3d_aux.hip", "fv3d_api.hip", "fv3d_batch_api.hip", "fv3d_adjoint.hip",
+SOURCES = ["fv3d_stage.hip", "fv3d_stage_w32.hip", "fv3d_stage_f32.hip", "fv3d_stage_f32_w32.hip",
+           "fv3d_batch.hip", "fv3d_batch_f32.hip", "fv3d_batch_f32_w32.hip", "fv3d_aux.hip", "fv3d_api.hip", "fv3d_batch_api.hip", "fv3d_adjoint.hip",
            "fv3d_adjoint_f32.hip", "fv3d_adjoint_api.hip", "fv3d_functors.hip",
            "fv3d_noalloc.hip"]
 # per-source extra flags.  The FP32 kernels flush FP32 denormals (FP64 keeps
@@ -43,7 +43,9 @@ F32 = ["-fno-hip-fp32-correctly-rounded-divide-sqrt", "-fgpu-flush-denormals-to-
        "-ffp-contract=on"]
 EXTRA = {
     "fv3d_stage_f32.hip": F32,
+    "fv3d_stage_f32_w32.hip": F32,
     "fv3d_batch_f32.hip": F32,
+    "fv3d_batch_f32_w32.hip": F32,
     "fv3d_adjoint_f32.hip": F32,
 }
 

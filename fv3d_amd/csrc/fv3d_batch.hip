@@ -1,12 +1,13 @@
-// fv3d_batch.hip -- FP64 instantiation of the persistent multi-box stage
-// kernel (fv3d_batch_impl.h) and the step-advance kernel.
+// fv3d_batch.hip -- FP64 instantiations of the persistent multi-box stage
+// kernel (fv3d_batch_impl.h), both tile widths, and the step-advance kernel.
 #include "fv3d_batch_impl.h"
 
 namespace fv3d_hip {
 
 template <>
 hipError_t launch_batch_stage<double>(const BatchLaunch& L, hipStream_t s) {
-  return batch_launch_mode<double, STAGE_PLAIN>(L, s);
+  if (L.wx == 32) return launch_batch_stage_wx<double, STAGE_PLAIN, 32>(L, s);
+  return launch_batch_stage_wx<double, STAGE_PLAIN, 64>(L, s);
 }
 
 // end of a step: advance the step counter and clear the queue counters of

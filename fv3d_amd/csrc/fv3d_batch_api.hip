@@ -48,6 +48,7 @@ struct fv3d_batch {
   int nitems = 0;
   int grid = 0;
   int nzc_cap = 0;  // z-chunk cap (0: per-box default)
+  int wx = 0;       // stage tile width of every box (0: stage_wx_for the ensemble)
   long step = 0;    // host mirror of the device step counter
   double* qbuf = nullptr;
   size_t qbuf_elems = 0;
@@ -82,12 +83,15 @@ static int build_worklist(fv3d_batch* b) {
   std::vector<BoxDesc> desc(n);
   std::vector<double> box_cost(n);
   std::vector<std::vector<std::pair<double, int2>>> box_items(n);
+  const bool fp64 = b->precision == FV3D_FP64;
+  // one width for the persistent kernel; the per-box loop takes each box's own
+  const int wx = b->wx ? b->wx : stage_wx_for(b->params.data(), n, fp64);
   for (int i = 0; i < n; ++i) {
     const fv3d_params& p = b->params[i];
     int nzc = b->nzc_cap;
     if (nzc <= 0) nzc = std::min(p.Nz, 32);
-    b->dp[i] = make_devparams(p, nzc, b->num_cus, b->precision == FV3D_FP64);
-    b->dp_loop[i] = make_devparams(p, 0, b->num_cus, b->precision == FV3D_FP64);
+    b->dp[i] = make_devparams(p, nzc, b->num_cus, fp64, wx);
+    b->dp_loop[i] = make_devparams(p, 0, b->num_cus, fp64);
     DevParams& d = b->dp[i];
     desc[i].P = d;
     desc[i].off = b->off[i];
@@ -333,6 +337,7 @@ static int enqueue_persistent(fv3d_batch* b) {
   L.riemann = p0.riemann_solver;
   L.tc = p0.thermal_conductivity_active != 0;
   L.visc = p0.viscosity_active != 0;
+  L.wx = b->dp[0].twx;
   // the lean variant only if every box allows it, else the one any box allows
   L.var = STAGE_VAR_LEAN;
   for (const DevParams& d : b->dp) {
@@ -476,7 +481,25 @@ int fv3d_batch_configure(fv3d_batch* b, int nzc_cap, int grid) {
   return build_worklist(b);
 }
 
+int fv3d_batch_set_tile_width(fv3d_batch* b, int wx) {
+  if (!b || (wx != 0 && wx != 32 && wx != 64)) return bfail(b, FV3D_EINVAL, "tile width must be 0, 32 or 64");
+  BCHK(b, hipSetDevice(b->device));
+  BCHK(b, hipStreamSynchronize(b->stream));
+  b->wx = wx;
+  return build_worklist(b);
+}
+
+int fv3d_batch_tile_width(const fv3d_batch* b) {
+  if (!b || b->dp.empty()) return -1;
+  return b->dp[0].twx;
+}
+
 int fv3d_batch_tune(fv3d_batch* b, int nsteps, const char* cache_path, int* nzc_out, int* grid_out) {
+  return fv3d_batch_tune_shape(b, nsteps, cache_path, nzc_out, grid_out, nullptr);
+}
+
+int fv3d_batch_tune_shape(fv3d_batch* b, int nsteps, const char* cache_path, int* nzc_out,
+                          int* grid_out, int* wx_out) {
   if (!b || nsteps <= 0) return bfail(b, FV3D_EINVAL, "bad argument");
   BCHK(b, hipSetDevice(b->device));
   // signature of the ensemble: shapes, precision and physics switches
@@ -489,16 +512,31 @@ int fv3d_batch_tune(fv3d_batch* b, int nsteps, const char* cache_path, int* nzc_
       << p0.time_stepping << "_c" << p0.thermal_conductivity_active << "_v" << p0.viscosity_active;
   for (auto& kv : shapes) sig << "_" << kv.first << ":" << kv.second;
   const std::string key = sig.str();
+  // the tile widths to try: both, or the one FV3D_TILE_WX fixes
+  std::vector<int> widths = {64, 32};
+  if (const char* e = std::getenv("FV3D_TILE_WX")) {
+    const int w = std::atoi(e);
+    if (w == 32 || w == 64) widths = {w};
+  }
+  auto finish = [&](int nzc, int grid, int wx) -> int {
+    if (nzc_out) *nzc_out = nzc;
+    if (grid_out) *grid_out = grid;
+    if (wx_out) *wx_out = wx;
+    b->wx = wx;
+    return fv3d_batch_configure(b, nzc, grid);
+  };
   if (cache_path && *cache_path) {
+    // lines "signature nzc grid wx"
     std::ifstream in(cache_path);
-    std::string k;
-    int nzc, grid;
-    while (in >> k >> nzc >> grid)
-      if (k == key) {
-        if (nzc_out) *nzc_out = nzc;
-        if (grid_out) *grid_out = grid;
-        return fv3d_batch_configure(b, nzc, grid);
-      }
+    std::string line;
+    while (std::getline(in, line)) {
+      std::istringstream ls(line);
+      std::string k;
+      int nzc, grid, wx;
+      if (!(ls >> k >> nzc >> grid >> wx) || k != key) continue;
+      if (std::find(widths.begin(), widths.end(), wx) == widths.end()) continue;
+      return finish(nzc, grid, wx);
+    }
   }
   // candidates: z-chunk cap x persistent grid size, timed on a copy of the state
   const size_t bytes = (size_t)b->total * b->elem;
@@ -527,27 +565,28 @@ int fv3d_batch_tune(fv3d_batch* b, int nsteps, const char* cache_path, int* nzc_
     b->step = step0;
     return FV3D_OK;
   };
-  int best_nzc = 0, best_grid = b->num_cus;
+  int best_nzc = 0, best_grid = b->num_cus, best_wx = widths[0];
   float best = 1e30f;
-  for (int nzc : {8, 16, 32, 64}) {
-    for (int mult : {1, 2, 4}) {
-      const int grid = b->num_cus * mult;
-      if (int rc = fv3d_batch_configure(b, nzc, grid)) return rc;
-      float ms = 0.f;
-      if (int rc = fv3d_batch_time(b, nsteps, FV3D_BATCH_GRAPH, &ms)) return rc;
-      if (int rc = restore()) return rc;
-      if (ms < best) { best = ms; best_nzc = nzc; best_grid = grid; }
+  for (int wx : widths) {
+    b->wx = wx;
+    for (int nzc : {8, 16, 32, 64}) {
+      for (int mult : {1, 2, 4}) {
+        const int grid = b->num_cus * mult;
+        if (int rc = fv3d_batch_configure(b, nzc, grid)) return rc;
+        float ms = 0.f;
+        if (int rc = fv3d_batch_time(b, nsteps, FV3D_BATCH_GRAPH, &ms)) return rc;
+        if (int rc = restore()) return rc;
+        if (ms < best) { best = ms; best_nzc = nzc; best_grid = grid; best_wx = wx; }
+      }
     }
   }
   for (void* s : saved)
     if (s) hipFree(s);
   if (cache_path && *cache_path) {
     std::ofstream outf(cache_path, std::ios::app);
-    outf << key << " " << best_nzc << " " << best_grid << "\n";
+    outf << key << " " << best_nzc << " " << best_grid << " " << best_wx << "\n";
   }
-  if (nzc_out) *nzc_out = best_nzc;
-  if (grid_out) *grid_out = best_grid;
-  return fv3d_batch_configure(b, best_nzc, best_grid);
+  return finish(best_nzc, best_grid, best_wx);
 }
 
 }  // extern "C"

@@ -49,8 +49,8 @@ __device__ __forceinline__ void stage_pointers(const BatchBufs& bb, long long of
   }
 }
 
-template <typename real, int TY, int RECON, int RS, bool TC, bool VISC, int MODE, int VAR>
-__global__ void __launch_bounds__(64 * (TY + 4))
+template <typename real, int TY, int RECON, int RS, bool TC, bool VISC, int MODE, int VAR, int WX>
+__global__ void __launch_bounds__(WX * (TY + 4))
 batch_stage_kernel(const BoxDesc* __restrict__ boxes, const int2* __restrict__ items,
                    const int* __restrict__ qstart, unsigned* counters,
                    const long long* __restrict__ step_ptr, BatchBufs bb, int stage, int ts,
@@ -90,44 +90,48 @@ batch_stage_kernel(const BoxDesc* __restrict__ boxes, const int2* __restrict__ i
     const unsigned long long* inv_cur = ring + (step % 3);
     unsigned long long* inv_next = last_stage ? ring + ((step + 1) % 3) : nullptr;
     unsigned long long* inv_zero = stage == 0 ? ring + ((step + 2) % 3) : nullptr;
-    stage_tile<real, TY, RECON, RS, TC, VISC, MODE, VAR>(Uin, Urk, Uout, Cio, inv_cur, inv_next,
-                                                         inv_zero, bb.health, P, wi.y);
+    stage_tile<real, TY, RECON, RS, TC, VISC, MODE, VAR, WX>(Uin, Urk, Uout, Cio, inv_cur,
+                                                             inv_next, inv_zero, bb.health, P,
+                                                             wi.y);
   }
 }
 
-template <typename real, int TY, int RECON, int RS, bool TC, bool VISC, int MODE, int VAR>
+template <typename real, int TY, int RECON, int RS, bool TC, bool VISC, int MODE, int VAR, int WX>
 static hipError_t batch_launch_t(const BatchLaunch& L, hipStream_t s) {
-  hipLaunchKernelGGL((batch_stage_kernel<real, TY, RECON, RS, TC, VISC, MODE, VAR>), dim3(L.grid),
-                     dim3(64 * (TY + 4)), 0, s, L.boxes, L.items, L.qstart, L.counters, L.step,
-                     L.bufs, L.stage, L.ts, L.last_stage);
+  hipLaunchKernelGGL((batch_stage_kernel<real, TY, RECON, RS, TC, VISC, MODE, VAR, WX>),
+                     dim3(L.grid), dim3(WX * (TY + 4)), 0, s, L.boxes, L.items, L.qstart,
+                     L.counters, L.step, L.bufs, L.stage, L.ts, L.last_stage);
   return hipGetLastError();
 }
 
-template <typename real, int RECON, int RS, int MODE, int VAR>
+template <typename real, int RECON, int RS, int MODE, int VAR, int WX>
 static hipError_t batch_launch_phys(const BatchLaunch& L, hipStream_t s) {
-  if (L.tc && L.visc) return batch_launch_t<real, stage_ty_visc<real>(), RECON, RS, true, true, MODE, VAR>(L, s);
-  if (L.tc) return batch_launch_t<real, STAGE_TY, RECON, RS, true, false, MODE, VAR>(L, s);
-  if (L.visc) return batch_launch_t<real, stage_ty_visc<real>(), RECON, RS, false, true, MODE, VAR>(L, s);
-  return batch_launch_t<real, STAGE_TY, RECON, RS, false, false, MODE, VAR>(L, s);
+  constexpr int TV = stage_ty_of<real, true, WX>(), TH = stage_ty_of<real, false, WX>();
+  if (L.tc && L.visc) return batch_launch_t<real, TV, RECON, RS, true, true, MODE, VAR, WX>(L, s);
+  if (L.tc) return batch_launch_t<real, TH, RECON, RS, true, false, MODE, VAR, WX>(L, s);
+  if (L.visc) return batch_launch_t<real, TV, RECON, RS, false, true, MODE, VAR, WX>(L, s);
+  return batch_launch_t<real, TH, RECON, RS, false, false, MODE, VAR, WX>(L, s);
 }
 
-// L.var: the stage variant every box allows (the ensemble rejects C91)
-template <typename real, int MODE>
-static hipError_t batch_launch_mode(const BatchLaunch& L, hipStream_t s) {
+// L.var: the stage variant every box allows (the ensemble rejects C91);
+// every box's tiles were sized for width WX (L.wx)
+template <typename real, int MODE, int WX>
+hipError_t launch_batch_stage_wx(const BatchLaunch& L, hipStream_t s) {
+  if (L.wx != WX) return hipErrorInvalidValue;
   const bool plm = L.recon == FV3D_PLM, hll = L.riemann == FV3D_HLL;
   if (L.var == STAGE_VAR_LEAN) {
-    if (hll) return batch_launch_phys<real, FV3D_PLM, FV3D_HLL, MODE, STAGE_VAR_LEAN>(L, s);
-    return batch_launch_phys<real, FV3D_PLM, FV3D_HLLC, MODE, STAGE_VAR_LEAN>(L, s);
+    if (hll) return batch_launch_phys<real, FV3D_PLM, FV3D_HLL, MODE, STAGE_VAR_LEAN, WX>(L, s);
+    return batch_launch_phys<real, FV3D_PLM, FV3D_HLLC, MODE, STAGE_VAR_LEAN, WX>(L, s);
   }
   if (L.var == STAGE_VAR_LEAN_ANYH) {
-    if (hll) return batch_launch_phys<real, FV3D_PLM, FV3D_HLL, MODE, STAGE_VAR_LEAN_ANYH>(L, s);
-    return batch_launch_phys<real, FV3D_PLM, FV3D_HLLC, MODE, STAGE_VAR_LEAN_ANYH>(L, s);
+    if (hll) return batch_launch_phys<real, FV3D_PLM, FV3D_HLL, MODE, STAGE_VAR_LEAN_ANYH, WX>(L, s);
+    return batch_launch_phys<real, FV3D_PLM, FV3D_HLLC, MODE, STAGE_VAR_LEAN_ANYH, WX>(L, s);
   }
   constexpr int G = STAGE_VAR_GENERAL;
-  if (plm && hll) return batch_launch_phys<real, FV3D_PLM, FV3D_HLL, MODE, G>(L, s);
-  if (plm) return batch_launch_phys<real, FV3D_PLM, FV3D_HLLC, MODE, G>(L, s);
-  if (hll) return batch_launch_phys<real, FV3D_PCM, FV3D_HLL, MODE, G>(L, s);
-  return batch_launch_phys<real, FV3D_PCM, FV3D_HLLC, MODE, G>(L, s);
+  if (plm && hll) return batch_launch_phys<real, FV3D_PLM, FV3D_HLL, MODE, G, WX>(L, s);
+  if (plm) return batch_launch_phys<real, FV3D_PLM, FV3D_HLLC, MODE, G, WX>(L, s);
+  if (hll) return batch_launch_phys<real, FV3D_PCM, FV3D_HLL, MODE, G, WX>(L, s);
+  return batch_launch_phys<real, FV3D_PCM, FV3D_HLLC, MODE, G, WX>(L, s);
 }
 
 }  // namespace fv3d_hip

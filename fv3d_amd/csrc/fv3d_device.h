@@ -31,6 +31,7 @@ struct DevParams {
   double bctc_zmin_value, bctc_zmax_value;
   double mu;
   double CFL, inv_tc, inv_visc; // dt inputs; inv_tc/inv_visc are constants of the run
+  int twx, tty;                 // stage tile: columns (64 or 32) and rows
   int tiles_x, tiles_y, tiles_z, nzc;
   int dt_direct;                // the dt slot holds dt itself (a frozen dt sequence)
   // boundary_z == BC_C91: primitives of the four z-ghost planes (k = -2, -1,

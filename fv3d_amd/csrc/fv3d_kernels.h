@@ -6,21 +6,18 @@
 
 namespace fv3d_hip {
 
-// rows per stage tile (the tile is 64 x STAGE_TY columns)
-constexpr int STAGE_TY = 8;
-// with viscosity: FP32 keeps 8 rows (12 waves, 168 VGPRs, no spills); FP64
-// takes 4 rows (8 waves, 231 of 256 VGPRs, no spills), where 8 rows would
-// spill about 80 VGPRs and run 27% slower on the stratified box
-constexpr int STAGE_TY_VISC_F64 = 4;
-constexpr int STAGE_TY_VISC_F32 = 8;
-template <typename real>
-constexpr int stage_ty_visc() {
-  return sizeof(real) == 8 ? STAGE_TY_VISC_F64 : STAGE_TY_VISC_F32;
+// Stage tile shapes: WX columns (64: one window row per wave, or 32: two
+// rows per wave) by stage_ty rows.  Without viscosity: 64 x 8 or 32 x 16
+// (12 or 10 waves, 168 VGPRs).  With viscosity, FP32 keeps those shapes
+// (no spills); FP64 takes 64 x 4 or 32 x 12 (8 waves, 256 VGPRs), where
+// 64 x 8 spills about 80 VGPRs and runs 27% slower on the stratified box.
+template <typename real, bool VISC, int WX>
+constexpr int stage_ty_of() {
+  return WX == 64 ? ((VISC && sizeof(real) == 8) ? 4 : 8) : ((VISC && sizeof(real) == 8) ? 12 : 16);
 }
-inline int stage_ty(bool visc, bool fp64) {
-  return visc ? (fp64 ? STAGE_TY_VISC_F64 : STAGE_TY_VISC_F32) : STAGE_TY;
+inline int stage_ty(bool visc, bool fp64, int wx) {
+  return wx == 64 ? ((visc && fp64) ? 4 : 8) : ((visc && fp64) ? 12 : 16);
 }
-constexpr int STAGE_WX = 64;
 
 // Stage kernel variants.  STAGE_VAR_GENERAL tests the optional features at
 // run time.  STAGE_VAR_C91 adds the precomputed C91 z ghosts.  The lean
@@ -60,6 +57,9 @@ struct StageLaunch {
 
 template <typename real>
 hipError_t launch_stage(const StageLaunch<real>& L, hipStream_t s);
+// the instantiations of one tile width (L.P.twx); launch_stage dispatches
+template <typename real, int MODE, int WX>
+hipError_t launch_stage_wx(const StageLaunch<real>& L, hipStream_t s);
 
 // ---- many boxes: one persistent launch per stage (fv3d_batch_impl.h) ----
 struct BoxDesc {
@@ -88,10 +88,13 @@ struct BatchLaunch {
   bool tc, visc;
   int mode;  // STAGE_PLAIN / STAGE_INC1 / STAGE_INC2
   int var = 0;  // stage kernel variant every box allows (STAGE_VAR_*)
+  int wx = 64;  // tile width of every box (DevParams::twx)
 };
 
 template <typename real>
 hipError_t launch_batch_stage(const BatchLaunch& L, hipStream_t s);
+template <typename real, int MODE, int WX>
+hipError_t launch_batch_stage_wx(const BatchLaunch& L, hipStream_t s);
 hipError_t launch_batch_advance(long long* step, unsigned* counters, int n, hipStream_t s);
 
 // ---- discrete adjoint of one stage (fv3d_adjoint_impl.h) ----

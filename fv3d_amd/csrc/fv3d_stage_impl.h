@@ -12,26 +12,27 @@
 // HLL/HLLC, flux divergence with gravity and the density floor, thermal
 // conduction and viscosity.
 //
-// Work layout.  A workgroup owns a tile of 64 (x) by TY (y) columns and
-// marches through a chunk of z planes.  Its window is (TY+4) rows by 68
-// columns (2 halo cells each side).  Waves 0..TY+3 own one window row each
-// (lane = x, coalesced loads); the four halo-row waves also own the 4 halo
-// columns of every row.  Every row owner keeps its cell's primitives for
-// planes k, k+1, k+2 in registers, so z neighbours, z slopes and z faces
-// never touch LDS.  TY+4 = 12 waves leave 168 VGPRs per lane (3 waves per
-// SIMD), which the FP64 solver needs without spilling.  Per
-// plane the owners publish plane k to LDS, waves compute each x and y face
-// once (the reference solves each face twice, Update.h:125-126, with
-// identical inputs), and the TY interior waves update their cells.
+// Work layout.  A workgroup owns a tile of WX (x, 64 or 32) by TY (y)
+// columns and marches through a chunk of z planes.  Its window is (TY+4)
+// rows by WX+4 columns (2 halo cells each side).  Every window row is owned
+// by WX lanes (x = lane, coalesced loads): one row per wave for 64-column
+// tiles, two rows per wave (lanes 0-31, 32-63) for 32-column tiles.  The
+// waves of the halo rows also own the 4 halo columns of every row.  Every
+// row owner keeps its cell's primitives for planes k, k+1, k+2 in
+// registers, so z neighbours, z slopes and z faces never touch LDS.  Twelve
+// waves (64 x 8) leave 168 VGPRs per lane (3 waves per SIMD), which the FP64
+// solver needs without spilling.  Per plane the owners publish plane k to
+// LDS, waves compute each x and y face once (the reference solves each face
+// twice, Update.h:125-126, with identical inputs), and the interior rows
+// update their cells.  The 32-column tile wastes no lanes on 32- and
+// 96-cell-wide boxes and re-converts fewer halo cells per interior cell
+// (36 x 20 window cells for 32 x 16, against 68 x 12 for 64 x 8).
 #pragma once
 #include "fv3d_device.h"
 #include "fv3d_kernels.h"
 #include "fv3d_physics.h"
 
 namespace fv3d_hip {
-
-constexpr int WX = 64;      // interior columns per tile (one wave)
-constexpr int NC = WX + 4;  // window columns
 
 // Stage modes.  STAGE_PLAIN: Unew = U + dt L(U) (+ RK2 combine with Urk).
 // FP32 SSP-RK2 with a compensated accumulation uses two further modes:
@@ -122,14 +123,21 @@ __device__ __forceinline__ void face_flux4(const Prim<real>& a, const Prim<real>
 // per workgroup) and by the persistent multi-box kernel (a work list of
 // (box, tile) items).
 template <typename real, int TY, int RECON, int RS, bool TC, bool VISC, int MODE,
-          int VAR = STAGE_VAR_GENERAL>
+          int VAR = STAGE_VAR_GENERAL, int WX = 64>
 __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const real* Urk,
                                            real* Uout, real* Cio,
                                            const unsigned long long* __restrict__ inv_cur,
                                            unsigned long long* inv_next,
                                            unsigned long long* inv_zero, int* health,
                                            const DevParams& P, const int tile) {
-  constexpr int NR = TY + 4;
+  constexpr int NR = TY + 4;   // window rows
+  constexpr int NC = WX + 4;   // window columns
+  constexpr int RPW = 64 / WX;  // window rows per wave
+  constexpr int NW = NR / RPW;  // waves
+  static_assert(WX == 64 || WX == 32, "tile width: one or two rows per wave");
+  // interior rows start and end on a wave boundary; every halo column fits
+  // the lanes given to it
+  static_assert(TY % RPW == 0 && NR <= WX, "tile rows");
   constexpr bool C91Z = VAR == STAGE_VAR_C91;
   constexpr bool LEAN = VAR == STAGE_VAR_LEAN || VAR == STAGE_VAR_LEAN_ANYH;
   constexpr bool POW2 = VAR == STAGE_VAR_LEAN;
@@ -161,10 +169,11 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
   // (a face's two cells read the same values in the same order)
   __shared__ real svx[VFACE ? 4 : 1][VFACE ? TY : 1][VFACE ? WX + 1 : 1];
   __shared__ real svy[VFACE ? 4 : 1][VFACE ? TY + 1 : 1][VFACE ? WX : 1];
-  __shared__ double sred[NR];
+  __shared__ double sred[NW];
 
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
+  const int xl = threadIdx.x % WX;  // interior column of the lane's row
 
   const int tx = tile % P.tiles_x;
   const int t2 = tile / P.tiles_x;
@@ -173,23 +182,32 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
   const int i0 = tx * WX, j0 = ty * TY, k0 = tz * P.nzc;
   const int k1 = min(k0 + P.nzc, P.Nz);
 
-  // every wave owns one window row (lane = column 2..65); the four halo-row
-  // waves also own, in lanes 0..NR-1, one of the four halo columns 0, 1,
-  // 66, 67 of every row
-  const int r = wave, c = lane + 2;
+  // every thread owns window cell (r, c), c = 2..WX+1.  The halo columns
+  // 0, 1, WX+2, WX+3 of every row: for 64-column tiles the four halo-row
+  // waves own one each in lanes 0..NR-1; for 32-column tiles the two
+  // halo-row waves own two each, in lanes 0..NR-1 and 32..32+NR-1.
+  const int r = threadIdx.x / WX, c = xl + 2;
   const int gi = i0 - 2 + c, gj = j0 - 2 + r;
   bool fx, fy;
   const int si = map_coord(gi, P.Nx, P.bcx, fx);
   const int sj = map_coord(gj, P.Ny, P.bcy, fy);
   const long long sxy = (long long)si + (long long)sj * P.sy;
 
-  const bool interior = wave >= 2 && wave < TY + 2;
-  const bool yworker = wave >= 2 && wave < TY + 3;
+  // wave-uniform: interior rows 2..TY+1 start and end on wave boundaries
+  const bool interior = r >= 2 && r < TY + 2;
+  const bool yworker = r >= 2 && r < TY + 3;
   const bool store = interior && gi < P.Nx && gj < P.Ny;
-  const int hidx = wave < 2 ? wave : wave - (NR - 4);
-  const bool hown = (wave < 2 || wave >= NR - 2) && lane < NR;
-  const int hr = lane < NR ? lane : 0;
+  const bool hwave = RPW == 1 ? (wave < 2 || wave >= NW - 2) : (wave == 0 || wave == NW - 1);
+  const int hidx = RPW == 1 ? (wave < 2 ? wave : wave - (NW - 4))
+                            : (wave == 0 ? 0 : 2) + (lane >> 5);
+  const int hl = RPW == 1 ? lane : (lane & 31);
+  const bool hown = hwave && hl < NR;
+  const int hr = hl < NR ? hl : 0;
   const int hc = hidx < 2 ? hidx : WX + hidx;
+  // the right x face of the last interior column, row lane+2: a wave that
+  // computes no other x face (the last for 64-column tiles, the first for
+  // 32-column tiles, whose last wave holds y faces)
+  const bool xedge = (RPW == 1 ? wave == NW - 1 : wave == 0) && lane < TY;
   bool hfx, hfy;
   const int hsi = map_coord(i0 - 2 + hc, P.Nx, P.bcx, hfx);
   const int hsj = map_coord(j0 - 2 + hr, P.Ny, P.bcy, hfy);
@@ -340,9 +358,9 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
       face_flux4<real, FV3D_IX, RECON, RS>(lds_prim(r, c - 2), lds_prim(r, c - 1), P0,
                                            lds_prim(r, c + 1), P, F, po);
 #pragma unroll
-      for (int n = 0; n < 5; ++n) sfx[n][r - 2][lane] = F[n];
-    } else if (wave == NR - 1 && lane < TY) {
-      // the right face of column 63 for row lane+2
+      for (int n = 0; n < 5; ++n) sfx[n][r - 2][xl] = F[n];
+    } else if (xedge) {
+      // the right face of the last interior column for row lane+2
       real F[5], po;
       face_flux4<real, FV3D_IX, RECON, RS>(lds_prim(lane + 2, WX), lds_prim(lane + 2, WX + 1),
                                            lds_prim(lane + 2, WX + 2),
@@ -356,7 +374,7 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
       face_flux4<real, FV3D_IY, RECON, RS>(lds_prim(r - 2, c), lds_prim(r - 1, c), P0,
                                            lds_prim(r + 1, c), P, F, po);
 #pragma unroll
-      for (int n = 0; n < 5; ++n) sfy[n][r - 2][lane] = F[n];
+      for (int n = 0; n < 5; ++n) sfy[n][r - 2][xl] = F[n];
     }
     // ---- z face k+1/2 from registers
     real Fzh[5] = {0, 0, 0, 0, 0}, pouth = 0;
@@ -387,8 +405,8 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
         real t[5];
         visc_x_side<real>(vstencil(r, c), 1, oodx, oody, oodz, t);
 #pragma unroll
-        for (int n = 1; n < 5; ++n) svx[n - 1][r - 2][lane] = mu * t[n];
-      } else if (wave == NR - 1 && lane < TY) {
+        for (int n = 1; n < 5; ++n) svx[n - 1][r - 2][xl] = mu * t[n];
+      } else if (xedge) {
         real t[5];
         visc_x_side<real>(vstencil(lane + 2, WX + 2), 1, oodx, oody, oodz, t);
 #pragma unroll
@@ -398,7 +416,7 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
         real t[5];
         visc_y_side<real>(vstencil(r, c), 1, oodx, oody, oodz, t);
 #pragma unroll
-        for (int n = 1; n < 5; ++n) svy[n - 1][r - 2][lane] = mu * t[n];
+        for (int n = 1; n < 5; ++n) svy[n - 1][r - 2][xl] = mu * t[n];
       }
     }
     __syncthreads();
@@ -411,12 +429,12 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
       for (int n = 0; n < 5; ++n) u[n] = ub[n];
 #pragma unroll
       for (int n = 0; n < 5; ++n) {
-        const real t = (sfx[n][r - 2][lane] - sfx[n][r - 2][lane + 1]) * dt;
+        const real t = (sfx[n][r - 2][xl] - sfx[n][r - 2][xl + 1]) * dt;
         u[n] = u[n] + over_h(t, p2x, P.idx, P.dx);
       }
 #pragma unroll
       for (int n = 0; n < 5; ++n) {
-        const real t = (sfy[n][r - 2][lane] - sfy[n][r - 1][lane]) * dt;
+        const real t = (sfy[n][r - 2][xl] - sfy[n][r - 1][xl]) * dt;
         u[n] = u[n] + over_h(t, p2y, P.idy, P.dy);
       }
       real zl[5], zh[5];
@@ -477,10 +495,10 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
         real vx[5] = {0, 0, 0, 0, 0}, vy[5] = {0, 0, 0, 0, 0}, vz[5] = {0, 0, 0, 0, 0};
 #pragma unroll
         for (int n = 1; n < 5; ++n) {
-          vx[n] += -svx[n - 1][r - 2][lane];
-          vx[n] += svx[n - 1][r - 2][lane + 1];
-          vy[n] += -svy[n - 1][r - 2][lane];
-          vy[n] += svy[n - 1][r - 1][lane];
+          vx[n] += -svx[n - 1][r - 2][xl];
+          vx[n] += svx[n - 1][r - 2][xl + 1];
+          vy[n] += -svy[n - 1][r - 2][xl];
+          vy[n] += svy[n - 1][r - 1][xl];
         }
         // z: both sides per cell (the reference's z branch differs between
         // a face's two cells)
@@ -560,15 +578,15 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
     __syncthreads();
     if (threadIdx.x == 0) {
       double m = 0.0;
-      for (int w = 0; w < NR; ++w) m = (sred[w] > m) ? sred[w] : m;
+      for (int w = 0; w < NW; ++w) m = (sred[w] > m) ? sred[w] : m;
       if (m > 0.0) atomicMax(inv_next, double_as_u64(m));
     }
     if (bad) atomicOr(health, 1);
   }
 }
 
-template <typename real, int TY, int RECON, int RS, bool TC, bool VISC, int MODE, int VAR>
-__global__ void __launch_bounds__(64 * (TY + 4))
+template <typename real, int TY, int RECON, int RS, bool TC, bool VISC, int MODE, int VAR, int WX>
+__global__ void __launch_bounds__(WX * (TY + 4))
 stage_kernel(const real* __restrict__ Uin, const real* Urk, real* Uout, real* Cio,
              const unsigned long long* __restrict__ inv_cur, unsigned long long* inv_next,
              unsigned long long* inv_zero, int* health, DevParams P) {
@@ -578,30 +596,33 @@ stage_kernel(const real* __restrict__ Uin, const real* Urk, real* Uout, real* Ci
   const int nb = gridDim.x, b = blockIdx.x;
   const int q8 = nb >> 3, r8 = nb & 7, xcd = b & 7, loc = b >> 3;
   const int tile = xcd * q8 + (xcd < r8 ? xcd : r8) + loc;
-  stage_tile<real, TY, RECON, RS, TC, VISC, MODE, VAR>(Uin, Urk, Uout, Cio, inv_cur, inv_next,
-                                                 inv_zero, health, P, tile);
+  stage_tile<real, TY, RECON, RS, TC, VISC, MODE, VAR, WX>(Uin, Urk, Uout, Cio, inv_cur, inv_next,
+                                                     inv_zero, health, P, tile);
 }
 
 // ---------------------------------------------------------------------------
 // launcher
 // ---------------------------------------------------------------------------
-template <typename real, int TY, int RECON, int RS, bool TC, bool VISC, int MODE, int VAR>
+template <typename real, int TY, int RECON, int RS, bool TC, bool VISC, int MODE, int VAR, int WX>
 static hipError_t launch_t(const StageLaunch<real>& L, hipStream_t s) {
   DevParams P = L.P;
+  // the tile counts were sized for this shape
+  if (P.twx != WX || P.tty != TY) return hipErrorInvalidValue;
   const int blocks = P.tiles_x * P.tiles_y * P.tiles_z;
-  hipLaunchKernelGGL((stage_kernel<real, TY, RECON, RS, TC, VISC, MODE, VAR>), dim3(blocks),
-                     dim3(64 * (TY + 4)), 0, s, L.Uin, L.Urk, L.Uout, L.Cio, L.inv_cur,
+  hipLaunchKernelGGL((stage_kernel<real, TY, RECON, RS, TC, VISC, MODE, VAR, WX>), dim3(blocks),
+                     dim3(WX * (TY + 4)), 0, s, L.Uin, L.Urk, L.Uout, L.Cio, L.inv_cur,
                      L.inv_next, L.inv_zero, L.health, P);
   return hipGetLastError();
 }
 
-template <typename real, int RECON, int RS, int MODE, int VAR>
+template <typename real, int RECON, int RS, int MODE, int VAR, int WX>
 static hipError_t launch_phys(const StageLaunch<real>& L, hipStream_t s) {
   const bool tc = L.tc, vi = L.visc;
-  if (tc && vi) return launch_t<real, stage_ty_visc<real>(), RECON, RS, true, true, MODE, VAR>(L, s);
-  if (tc) return launch_t<real, STAGE_TY, RECON, RS, true, false, MODE, VAR>(L, s);
-  if (vi) return launch_t<real, stage_ty_visc<real>(), RECON, RS, false, true, MODE, VAR>(L, s);
-  return launch_t<real, STAGE_TY, RECON, RS, false, false, MODE, VAR>(L, s);
+  constexpr int TV = stage_ty_of<real, true, WX>(), TH = stage_ty_of<real, false, WX>();
+  if (tc && vi) return launch_t<real, TV, RECON, RS, true, true, MODE, VAR, WX>(L, s);
+  if (tc) return launch_t<real, TH, RECON, RS, true, false, MODE, VAR, WX>(L, s);
+  if (vi) return launch_t<real, TV, RECON, RS, false, true, MODE, VAR, WX>(L, s);
+  return launch_t<real, TH, RECON, RS, false, false, MODE, VAR, WX>(L, s);
 }
 
 // fillC91 (BoundaryConditions.h:86-134) for the z-ghost planes of every
@@ -653,8 +674,8 @@ k_c91_ghosts(const real* __restrict__ U, real* g, DevParams P) {
   for (int n = 0; n < 5; ++n) d[n * P.sz] = res[n];
 }
 
-template <typename real, int MODE>
-static hipError_t launch_mode(const StageLaunch<real>& L, hipStream_t s) {
+template <typename real, int MODE, int WX>
+hipError_t launch_stage_wx(const StageLaunch<real>& L, hipStream_t s) {
   if (L.P.bcz == FV3D_BC_C91) {
     const long long n = 4 * L.P.sz;
     hipLaunchKernelGGL((k_c91_ghosts<real>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
@@ -668,23 +689,23 @@ static hipError_t launch_mode(const StageLaunch<real>& L, hipStream_t s) {
                 H = STAGE_VAR_LEAN_ANYH;
   const int var = stage_variant(L.P, L.recon);
   if (var == C) {
-    if (plm && hll) return launch_phys<real, FV3D_PLM, FV3D_HLL, MODE, C>(L, s);
-    if (plm) return launch_phys<real, FV3D_PLM, FV3D_HLLC, MODE, C>(L, s);
-    if (hll) return launch_phys<real, FV3D_PCM, FV3D_HLL, MODE, C>(L, s);
-    return launch_phys<real, FV3D_PCM, FV3D_HLLC, MODE, C>(L, s);
+    if (plm && hll) return launch_phys<real, FV3D_PLM, FV3D_HLL, MODE, C, WX>(L, s);
+    if (plm) return launch_phys<real, FV3D_PLM, FV3D_HLLC, MODE, C, WX>(L, s);
+    if (hll) return launch_phys<real, FV3D_PCM, FV3D_HLL, MODE, C, WX>(L, s);
+    return launch_phys<real, FV3D_PCM, FV3D_HLLC, MODE, C, WX>(L, s);
   }
   if (var == N) {
-    if (hll) return launch_phys<real, FV3D_PLM, FV3D_HLL, MODE, N>(L, s);
-    return launch_phys<real, FV3D_PLM, FV3D_HLLC, MODE, N>(L, s);
+    if (hll) return launch_phys<real, FV3D_PLM, FV3D_HLL, MODE, N, WX>(L, s);
+    return launch_phys<real, FV3D_PLM, FV3D_HLLC, MODE, N, WX>(L, s);
   }
   if (var == H) {
-    if (hll) return launch_phys<real, FV3D_PLM, FV3D_HLL, MODE, H>(L, s);
-    return launch_phys<real, FV3D_PLM, FV3D_HLLC, MODE, H>(L, s);
+    if (hll) return launch_phys<real, FV3D_PLM, FV3D_HLL, MODE, H, WX>(L, s);
+    return launch_phys<real, FV3D_PLM, FV3D_HLLC, MODE, H, WX>(L, s);
   }
-  if (plm && hll) return launch_phys<real, FV3D_PLM, FV3D_HLL, MODE, G>(L, s);
-  if (plm) return launch_phys<real, FV3D_PLM, FV3D_HLLC, MODE, G>(L, s);
-  if (hll) return launch_phys<real, FV3D_PCM, FV3D_HLL, MODE, G>(L, s);
-  return launch_phys<real, FV3D_PCM, FV3D_HLLC, MODE, G>(L, s);
+  if (plm && hll) return launch_phys<real, FV3D_PLM, FV3D_HLL, MODE, G, WX>(L, s);
+  if (plm) return launch_phys<real, FV3D_PLM, FV3D_HLLC, MODE, G, WX>(L, s);
+  if (hll) return launch_phys<real, FV3D_PCM, FV3D_HLL, MODE, G, WX>(L, s);
+  return launch_phys<real, FV3D_PCM, FV3D_HLLC, MODE, G, WX>(L, s);
 }
 
 }  // namespace fv3d_hip

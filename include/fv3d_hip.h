@@ -267,11 +267,22 @@ long long fv3d_batch_cells(const fv3d_batch* b);
 /* launch shape of the persistent kernels: z-chunk cap per work item (0:
  * min(Nz, 32)) and workgroups per launch (0: one per CU). */
 int fv3d_batch_configure(fv3d_batch* b, int nzc_cap, int grid);
-/* Autotuner: times nsteps graph steps for each z-chunk cap in {8,16,32,64}
- * and grid in {1,2,4} x CUs on a copy of the current state (restored
- * afterwards), then configures the fastest.  cache_path (may be NULL) is a
- * text file of "signature nzc grid" lines: a hit skips the search, a miss
- * appends the result. */
+/* Stage tile width of every box in the persistent kernels: 64 or 32
+ * columns, or 0 for the default (32 when 64-column tiles would cover at
+ * least 10% more cells over the ensemble, else 64).  Results do not depend
+ * on it.  No reference counterpart: the reference has no ensemble. */
+int fv3d_batch_set_tile_width(fv3d_batch* b, int wx);
+/* the tile width in use (-1 on a NULL batch) */
+int fv3d_batch_tile_width(const fv3d_batch* b);
+/* Autotuner: times nsteps graph steps for each tile width in {64, 32}, z-chunk
+ * cap in {8,16,32,64} and grid in {1,2,4} x CUs on a copy of the current
+ * state (restored afterwards), then configures the fastest.  cache_path (may
+ * be NULL) is a text file of "signature nzc grid wx" lines: a hit skips the
+ * search, a miss appends the result.  FV3D_TILE_WX=64 or 32 in the
+ * environment fixes the width. */
+int fv3d_batch_tune_shape(fv3d_batch* b, int nsteps, const char* cache_path, int* nzc_out,
+                          int* grid_out, int* wx_out);
+/* fv3d_batch_tune_shape without the width output */
 int fv3d_batch_tune(fv3d_batch* b, int nsteps, const char* cache_path, int* nzc_out,
                     int* grid_out);
 

@@ -34,7 +34,9 @@ def single_run(p, Q0, precision, nsteps):
 def batch_run(ens, precision, nsteps, mode, configure=None):
     with F.Batch([p for p, _ in ens], precision) as b:
         if configure:
-            b.configure(*configure)
+            b.set_tile_width(configure[2])
+            b.configure(*configure[:2])
+            assert b.tile_width == configure[2]
         for i, (_, Q0) in enumerate(ens):
             b.set_prim(i, Q0)
         dts = []
@@ -60,8 +62,8 @@ def test_batch_rejects_mixed_numerics():
 @pytest.mark.gpu
 @pytest.mark.parametrize("precision", [F.FP64, F.FP32, F.FP32_PLAIN])
 @pytest.mark.parametrize("mode", [F.BATCH_GRAPH, F.BATCH_PERSISTENT, F.BATCH_LOOP])
-def test_batch_equals_single_contexts(gpu_available, precision, mode):
-    print(f"batch vs single: precision={precision} mode={mode}", flush=True)
+def test_batch_equals_single_contexts(gpu_available, precision, mode, tile_wx):
+    print(f"batch vs single: precision={precision} mode={mode} wx={tile_wx}", flush=True)
     ens = problems.rt_ensemble(rt_base(), 6, seed=7, sides=SMALL_SIDES)
     nsteps = 6
     Qb, dtb = batch_run(ens, precision, nsteps, mode)
@@ -73,7 +75,7 @@ def test_batch_equals_single_contexts(gpu_available, precision, mode):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("configure", [(8, 0), (64, 1024), (1, 300)])
+@pytest.mark.parametrize("configure", [(8, 0, 64), (64, 1024, 32), (1, 300, 64), (3, 0, 32)])
 def test_batch_launch_shape_does_not_change_results(gpu_available, configure):
     print(f"batch launch shape {configure}", flush=True)
     ens = problems.rt_ensemble(rt_base(), 5, seed=11, sides=SMALL_SIDES)
@@ -85,7 +87,26 @@ def test_batch_launch_shape_does_not_change_results(gpu_available, configure):
 
 
 @pytest.mark.gpu
-def test_batch_fp64_boxes_match_oracle(gpu_available):
+def test_batch_default_tile_width_follows_cover(gpu_available, monkeypatch):
+    """Without a setting the ensemble takes 32-column tiles when 64-column
+    tiles would cover at least 10% more cells (overhang does no work)."""
+    monkeypatch.delenv("FV3D_TILE_WX", raising=False)
+    for sides, wx in (((32, 32), 32), ((96, 96), 32), ((64, 64), 64), ((48, 48), 64),
+                      ((256, 8), 64)):
+        ps = [rt_base(Nx=sides[0], Ny=sides[1], Nz=8)] * 2
+        with F.Batch(ps, F.FP32) as b:
+            assert b.tile_width == wx, sides
+            b.set_tile_width(64 if wx == 32 else 32)
+            assert b.tile_width != wx
+            b.set_tile_width(0)
+            assert b.tile_width == wx
+            with pytest.raises(F.FV3DError) as e:
+                b.set_tile_width(48)
+            assert e.value.code == 1
+
+
+@pytest.mark.gpu
+def test_batch_fp64_boxes_match_oracle(gpu_available, tile_wx):
     ens = problems.rt_ensemble(rt_base(), 3, seed=3, sides=(16, 24))
     nsteps = 5
     Qb, dtb = batch_run(ens, F.FP64, nsteps, F.BATCH_GRAPH)
@@ -151,7 +172,8 @@ def test_batch_tune_restores_state(gpu_available, tmp_path):
         before = [b.get_prim(i) for i in range(len(ens))]
         dt0 = b.compute_dt()
         nzc, grid = b.tune(nsteps=2, cache=cache)
-        assert nzc in (8, 16, 32, 64) and grid > 0
+        wx = b.tile_width
+        assert nzc in (8, 16, 32, 64) and grid > 0 and wx in (32, 64)
         assert b.step_count == 2
         assert np.array_equal(b.compute_dt(), dt0)
         for i in range(len(ens)):
@@ -165,6 +187,7 @@ def test_batch_tune_restores_state(gpu_available, tmp_path):
             b.set_prim(i, Q0)
         b.step(2)
         assert b.tune(nsteps=2, cache=cache) == (nzc, grid)
+        assert b.tile_width == wx
         b.step(3)
         for i in range(len(ens)):
             assert np.array_equal(b.get_prim(i), after[i])
