@@ -75,8 +75,8 @@ def read_valu_per_cell():
             txt = f.read()
     except OSError:
         return None
-    m = re.search(r"\| `fv3d_hip::stage_kernel<double, 8, 2, 1, false, false, 0, 2>` \| [0-9.e+]+ \| ([0-9]+) \|",
-                  txt)
+    sq = txt.split("## SQ counters", 1)[-1]
+    m = re.search(r"\| `fv3d_hip::stage_kernel<double, 8, 2, 1, false, false, 0, 2, 64>` \| ([0-9]+) \|", sq)
     return float(m.group(1)) if m else None
 
 
@@ -233,7 +233,7 @@ def fp32_error_vs_fp64(F, p, Q0, steps):
     return out
 
 
-def ensemble_form(F, nbox, steps, tune_cache=None):
+def ensemble_form(F, nbox, steps, tune_cache=None, loop=True):
     """The batched ensemble: nbox Rayleigh-Taylor boxes (sides drawn from
     {32,48,64,96} per axis, gamma ~ U[1.3,1.7], g=-0.1, periodic x/y,
     reflecting z), PLM + HLLC, SSP-RK2, CFL 0.8, FP32 with Kahan-compensated
@@ -261,10 +261,12 @@ def ensemble_form(F, nbox, steps, tune_cache=None):
     ms_graph = b32.time(steps, F.BATCH_GRAPH)
     b32.compute_dt()  # raises on non-finite state
     print(f"ensemble: graph {ms_graph:.1f} ms / {steps} steps", file=sys.stderr, flush=True)
-    bl = fresh(F.FP32)
-    ms_loop = bl.time(steps, F.BATCH_LOOP)
-    bl.close()
-    print(f"ensemble: loop {ms_loop:.1f} ms / {steps} steps", file=sys.stderr, flush=True)
+    ms_loop = None
+    if loop:
+        bl = fresh(F.FP32)
+        ms_loop = bl.time(steps, F.BATCH_LOOP)
+        bl.close()
+        print(f"ensemble: loop {ms_loop:.1f} ms / {steps} steps", file=sys.stderr, flush=True)
     b64 = fresh(F.FP64)
     b64.step(b32.step_count, F.BATCH_GRAPH)  # the graph run's capture step is one more
     b64.compute_dt()
@@ -277,10 +279,12 @@ def ensemble_form(F, nbox, steps, tune_cache=None):
     b32.close()
     b64.close()
     g = cells * steps / (ms_graph * 1e-3) / 1e9
-    lo = cells * steps / (ms_loop * 1e-3) / 1e9
+    lo = cells * steps / (ms_loop * 1e-3) / 1e9 if ms_loop else None
     return {"boxes": nbox, "cells": cells, "steps": steps, "l1_after_steps": b32_steps, "tuned_tile_width": wx, "tuned_nzc": nzc, "tuned_grid": grid,
-            "graph_Gcells": round(g, 4), "loop_Gcells": round(lo, 4), "speedup_vs_loop": round(g / lo, 3),
-            "graph_ms_per_step": round(ms_graph / steps, 4), "loop_ms_per_step": round(ms_loop / steps, 4),
+            "graph_Gcells": round(g, 4), "loop_Gcells": round(lo, 4) if lo else None,
+            "speedup_vs_loop": round(g / lo, 3) if lo else None,
+            "graph_ms_per_step": round(ms_graph / steps, 4),
+            "loop_ms_per_step": round(ms_loop / steps, 4) if ms_loop else None,
             "alg_bytes_per_cell_update": 140.0,
             "roofline_frac_graph": round(140.0 * g / HBM_PEAK_GBS, 4),
             "l1_rho_fp32_vs_fp64_max": max(l1), "l1_rho_fp32_vs_fp64_mean": float(np.mean(l1))}
@@ -478,6 +482,8 @@ def main():
     ap.add_argument("--forms", type=int, default=1, help="also time the FP32 forms")
     ap.add_argument("--ensemble", type=int, default=512, help="boxes of the batched ensemble (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=500)
+    ap.add_argument("--ensemble-loop", type=int, default=1,
+                    help="also time the per-box launch loop the ensemble replaces")
     ap.add_argument("--stratified", type=int, default=2000, help="steps of the stratified-convection form (0: skip)")
     ap.add_argument("--stratified-window", type=int, default=200,
                     help="steps of its device-only timing window")
@@ -536,7 +542,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "kernel": "fv3d_hip::stage_kernel<double,8,PLM,HLLC,false,false,STAGE_PLAIN,STAGE_VAR_LEAN>",
+            "kernel": "fv3d_hip::stage_kernel<double,8,PLM,HLLC,false,false,STAGE_PLAIN,STAGE_VAR_LEAN,64>",
             "alg_bytes_per_launch": bytes_per_launch,
             "mean_launch_ms": round(mean_launch_ms, 5),
             "per_stage": per_stage(stage_ms, cells, STAGE_BYTES["f64"]),
@@ -561,7 +567,8 @@ def main():
             "fp32_error_vs_fp64": fp32_error_vs_fp64(F, p, Q0, min(60, args.steps + args.warmup)),
         }
     if args.ensemble > 0:
-        out.setdefault("forms", {})["fp32_ensemble_rt"] = ensemble_form(F, args.ensemble, args.ensemble_steps)
+        out.setdefault("forms", {})["fp32_ensemble_rt"] = ensemble_form(F, args.ensemble, args.ensemble_steps,
+                                                                        loop=bool(args.ensemble_loop))
     if args.stratified > 0:
         out.setdefault("forms", {})["stratified_tc_visc"] = stratified_form(F, args.stratified,
                                                                              args.stratified_window)

@@ -569,7 +569,7 @@ int fv3d_batch_tune_shape(fv3d_batch* b, int nsteps, const char* cache_path, int
   float best = 1e30f;
   for (int wx : widths) {
     b->wx = wx;
-    for (int nzc : {8, 16, 32, 64}) {
+    for (int nzc : {8, 16, 32, 64, 128}) {
       for (int mult : {1, 2, 4}) {
         const int grid = b->num_cus * mult;
         if (int rc = fv3d_batch_configure(b, nzc, grid)) return rc;

@@ -275,7 +275,7 @@ int fv3d_batch_set_tile_width(fv3d_batch* b, int wx);
 /* the tile width in use (-1 on a NULL batch) */
 int fv3d_batch_tile_width(const fv3d_batch* b);
 /* Autotuner: times nsteps graph steps for each tile width in {64, 32}, z-chunk
- * cap in {8,16,32,64} and grid in {1,2,4} x CUs on a copy of the current
+ * cap in {8,16,32,64,128} and grid in {1,2,4} x CUs on a copy of the current
  * state (restored afterwards), then configures the fastest.  cache_path (may
  * be NULL) is a text file of "signature nzc grid wx" lines: a hit skips the
  * search, a miss appends the result.  FV3D_TILE_WX=64 or 32 in the
