@@ -188,6 +188,22 @@ def test_tiled_adjoint_equals_per_face_form(gpu_available, name, monkeypatch):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("name", ["periodic_hllc_plm_rk2_tc_visc", "walls_hll_plm_rk2_fixedT"])
+def test_adjoint_independent_of_tile_width(gpu_available, name, monkeypatch):
+    """The forward passes inside the adjoint run at 64- or 32-column stage
+    tiles (the same bits); the gradient differs only by the order of its
+    atomic sums."""
+    out = {}
+    for wx in (64, 32):
+        monkeypatch.setenv("FV3D_TILE_WX", str(wx))
+        pr = Problem(base_params(N=20, **CONFIGS[name]), nsteps=6)
+        out[wx] = pr.adjoint(K=3)
+    (_, g64, k64, m64), (_, g32, k32, m32) = out[64], out[32]
+    assert np.abs(g32 - g64).max() <= 1e-12 * np.abs(g64).max()
+    assert k32 == pytest.approx(k64, rel=1e-12, abs=1e-300) and m32 == pytest.approx(m64, rel=1e-12, abs=1e-300)
+
+
+@pytest.mark.gpu
 def test_adjoint_fp32_against_fp64(gpu_available):
     pr = Problem(base_params(N=24), nsteps=12)
     _, g64, k64, m64 = pr.adjoint(precision=F.FP64)

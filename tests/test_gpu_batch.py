@@ -120,7 +120,7 @@ def test_batch_fp64_boxes_match_oracle(gpu_available, tile_wx):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("ts", [F.TS_EULER, F.TS_RK2])
-def test_batch_sweep_kappa_mu_gamma(gpu_available, ts):
+def test_batch_sweep_kappa_mu_gamma(gpu_available, ts, tile_wx):
     """Per-box conduction and viscosity coefficients and gamma (a parameter
     sweep), with fixed-temperature walls; Euler and RK2."""
     print(f"batch sweep ts={ts}", flush=True)
