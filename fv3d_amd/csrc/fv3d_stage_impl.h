@@ -118,6 +118,32 @@ __device__ __forceinline__ void face_flux4(const Prim<real>& a, const Prim<real>
   unrotate_flux<real, DIR>(F);
 }
 
+// face_flux4 along x or along y, chosen per lane (isy): on every lane the
+// same operations on the same values as face_flux4<real, FV3D_IX or
+// FV3D_IY, ...>; the rotation into the face frame and back is a select.
+template <typename real, int RECON, int RS>
+__device__ __forceinline__ void face_flux_xy(const Prim<real>& a, const Prim<real>& b,
+                                             const Prim<real>& c, const Prim<real>& d, bool isy,
+                                             const DevParams& P, real F[5], real& pout) {
+  Prim<real> qL, qR;
+  if (RECON == FV3D_PLM) {
+    const Prim<real> sb = slope3<real, false>(a, b, c);
+    const Prim<real> sc = slope3<real, false>(b, c, d);
+    qL = plus_face(b, sb);
+    qR = minus_face(c, sc);
+  } else {
+    qL = b;
+    qR = c;
+  }
+  auto rot = [&](const Prim<real>& q) {
+    return Prim<real>{q.r, isy ? q.v : q.u, isy ? q.u : q.v, q.w, q.p};
+  };
+  riemann<real, RS>(rot(qL), rot(qR), (real)P.gamma, (real)P.gm1, (real)P.entho, F, pout);
+  const real f1 = F[1], f2 = F[2];
+  F[1] = isy ? f2 : f1;
+  F[2] = isy ? f1 : f2;
+}
+
 // One tile of one box: the z chunk `tile` (x fastest, then y, then z chunk)
 // marched by the whole workgroup.  Called by the single-box kernel (one tile
 // per workgroup) and by the persistent multi-box kernel (a work list of
@@ -204,10 +230,14 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
   const bool hown = hwave && hl < NR;
   const int hr = hl < NR ? hl : 0;
   const int hc = hidx < 2 ? hidx : WX + hidx;
-  // the right x face of the last interior column, row lane+2: a wave that
-  // computes no other x face (the last for 64-column tiles, the first for
-  // 32-column tiles, whose last wave holds y faces)
-  const bool xedge = (RPW == 1 ? wave == NW - 1 : wave == 0) && lane < TY;
+  // the right x face of the last interior column, row lane+2.  64-column
+  // tiles: the last wave, which computes no other face.  32-column tiles
+  // (MERGE): the last wave, lanes 32..32+TY-1, in the same face evaluation
+  // as the top row of y faces in its lanes 0-31
+  constexpr bool MERGE = RPW == 2;
+  static_assert(!MERGE || TY <= 32, "the edge faces take lanes 32..32+TY-1");
+  const bool xedge = !MERGE && wave == NW - 1 && lane < TY;
+  const bool mwave = MERGE && wave == NW - 1;
   bool hfx, hfy;
   const int hsi = map_coord(i0 - 2 + hc, P.Nx, P.bcx, hfx);
   const int hsj = map_coord(j0 - 2 + hr, P.Ny, P.bcy, hfy);
@@ -368,8 +398,29 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
 #pragma unroll
       for (int n = 0; n < 5; ++n) sfx[n][lane][WX] = F[n];
     }
+    if (mwave) {
+      // top y faces (lanes 0-31: row TY+2) and edge x faces (lanes 32..:
+      // rows 2..TY+1) in one face evaluation
+      const bool isy = lane < 32;
+      const int er = lane - 32 + 2;
+      if (isy || er < TY + 2) {
+        const int r0 = isy ? r - 2 : er, c0 = isy ? c : WX;
+        const int dr = isy ? 1 : 0, dc = isy ? 0 : 1;
+        real F[5], po;
+        face_flux_xy<real, RECON, RS>(lds_prim(r0, c0), lds_prim(r0 + dr, c0 + dc),
+                                      lds_prim(r0 + 2 * dr, c0 + 2 * dc),
+                                      lds_prim(r0 + 3 * dr, c0 + 3 * dc), isy, P, F, po);
+        if (isy) {
+#pragma unroll
+          for (int n = 0; n < 5; ++n) sfy[n][r - 2][xl] = F[n];
+        } else {
+#pragma unroll
+          for (int n = 0; n < 5; ++n) sfx[n][er - 2][WX] = F[n];
+        }
+      }
+    }
     // ---- y faces: face between window rows r-1 and r
-    if (yworker) {
+    if (yworker && !mwave) {
       real F[5], po;
       face_flux4<real, FV3D_IY, RECON, RS>(lds_prim(r - 2, c), lds_prim(r - 1, c), P0,
                                            lds_prim(r + 1, c), P, F, po);
@@ -411,6 +462,11 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
         visc_x_side<real>(vstencil(lane + 2, WX + 2), 1, oodx, oody, oodz, t);
 #pragma unroll
         for (int n = 1; n < 5; ++n) svx[n - 1][lane][WX] = mu * t[n];
+      } else if (mwave && lane >= 32 && lane - 32 < TY) {
+        real t[5];
+        visc_x_side<real>(vstencil(lane - 30, WX + 2), 1, oodx, oody, oodz, t);
+#pragma unroll
+        for (int n = 1; n < 5; ++n) svx[n - 1][lane - 32][WX] = mu * t[n];
       }
       if (yworker) {
         real t[5];
