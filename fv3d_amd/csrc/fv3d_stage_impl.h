@@ -233,10 +233,11 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
   // the right x face of the last interior column, row lane+2.  64-column
   // tiles: the last wave, which computes no other face.  32-column tiles
   // (MERGE): the last wave, lanes 32..32+TY-1, in the same face evaluation
-  // as the top row of y faces in its lanes 0-31
-  constexpr bool MERGE = RPW == 2;
+  // as the top row of y faces in its lanes 0-31; with viscosity in FP64
+  // (where the merged evaluation spills 7 more VGPRs) wave 0, lanes < TY
+  constexpr bool MERGE = RPW == 2 && !(VISC && sizeof(real) == 8);
   static_assert(!MERGE || TY <= 32, "the edge faces take lanes 32..32+TY-1");
-  const bool xedge = !MERGE && wave == NW - 1 && lane < TY;
+  const bool xedge = (RPW == 1 ? wave == NW - 1 : !MERGE && wave == 0) && lane < TY;
   const bool mwave = MERGE && wave == NW - 1;
   bool hfx, hfy;
   const int hsi = map_coord(i0 - 2 + hc, P.Nx, P.bcx, hfx);
