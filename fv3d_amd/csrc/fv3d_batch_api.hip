@@ -540,14 +540,20 @@ int fv3d_batch_tune_shape(fv3d_batch* b, int nsteps, const char* cache_path, int
   }
   // candidates: z-chunk cap x persistent grid size, timed on a copy of the state
   const size_t bytes = (size_t)b->total * b->elem;
-  std::vector<void*> saved;
+  // copies of the state buffers, freed on every return (errors included)
+  struct Saved {
+    std::vector<void*> p;
+    ~Saved() {
+      for (void* s : p)
+        if (s) hipFree(s);
+    }
+    void* operator[](int i) const { return p[i]; }
+  } saved;
   for (void* p : {b->A, b->B, b->C, b->comp}) {
     void* s = nullptr;
-    if (p) {
-      BCHK(b, hipMalloc(&s, bytes));
-      BCHK(b, hipMemcpyAsync(s, p, bytes, hipMemcpyDeviceToDevice, b->stream));
-    }
-    saved.push_back(s);
+    if (p) BCHK(b, hipMalloc(&s, bytes));
+    saved.p.push_back(s);
+    if (p) BCHK(b, hipMemcpyAsync(s, p, bytes, hipMemcpyDeviceToDevice, b->stream));
   }
   std::vector<unsigned long long> rings(4 * b->nbox);
   long long step_dev = 0;
@@ -580,8 +586,6 @@ int fv3d_batch_tune_shape(fv3d_batch* b, int nsteps, const char* cache_path, int
       }
     }
   }
-  for (void* s : saved)
-    if (s) hipFree(s);
   if (cache_path && *cache_path) {
     std::ofstream outf(cache_path, std::ios::app);
     outf << key << " " << best_nzc << " " << best_grid << " " << best_wx << "\n";
