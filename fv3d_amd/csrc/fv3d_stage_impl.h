@@ -281,9 +281,33 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
     for (int n = 0; n < 5; ++n) h[n] = Uin[base + n * P.sf];
   };
 
+  // raw values of the halo cell at plane kk, or its C91 ghost primitives,
+  // and their primitives with the reflections applied
+  auto fetch_halo_any = [&](int kk, real h[5]) {
+    if (C91Z && (kk < 0 || kk >= P.Nz)) {
+      const Prim<real> g = c91_ghost(kk, hxy, hfx, hfy);
+      h[0] = g.r; h[1] = g.u; h[2] = g.v; h[3] = g.w; h[4] = g.p;
+      return;
+    }
+    fetch_halo(kk, h);
+  };
+  auto halo_prim = [&](int kk, const real h[5]) -> Prim<real> {
+    if (C91Z && (kk < 0 || kk >= P.Nz)) return Prim<real>{h[0], h[1], h[2], h[3], h[4]};
+    bool hfz;
+    map_coord(kk, P.Nz, P.bcz, hfz);
+    Prim<real> q = cons_to_prim(h[0], h[1], h[2], h[3], h[4], gm1);
+    if (hfx) q.u = -q.u;
+    if (hfy) q.v = -q.v;
+    if (hfz) q.w = -q.w;
+    return q;
+  };
+
   // Registers carried from plane to plane.  Interior waves: cz = z slope at
   // k, fz/pz = z face k-1/2 and its pressure.  Halo waves: cz = conserved
-  // state of their halo cell at the next plane (prefetched one plane ahead).
+  // state of their halo cell at the next plane (prefetched one plane ahead);
+  // with viscosity, which also needs the halo cell's velocities at k-1 and
+  // k+1, cz = raw values of plane k+1 and fz = primitives of plane k, so no
+  // load waits before the plane is published.
   real cz[5] = {0, 0, 0, 0, 0}, fz[5] = {0, 0, 0, 0, 0}, pz = 0;
 
   // ---- warm-up: planes k0-2 .. k0+1, z slope at k0 and the z face k0-1/2
@@ -298,7 +322,17 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
     }
     face_flux4<real, FV3D_IZ, RECON, RS>(Pm2, Pm1, P0, P1, P, fz, pz);
   } else if (hown) {
-    fetch_halo(k0, cz);
+    if (VISC) {
+      // halo column ring: fz = primitives of plane k, cz = raw values of
+      // plane k+1, and the cell's LDS slot holds the velocities of plane k-1
+      const Prim<real> A = load_halo(k0);
+      fz[0] = A.r; fz[1] = A.u; fz[2] = A.v; fz[3] = A.w; fz[4] = A.p;
+      const Prim<real> B = load_halo(k0 - 1);
+      sqb[0][1][hr][hc] = B.u; sqb[0][2][hr][hc] = B.v; sqb[0][3][hr][hc] = B.w;
+      fetch_halo_any(k0 + 1, cz);
+    } else {
+      fetch_halo(k0, cz);
+    }
   }
   real Tm = 0, T0 = 0, T1 = 0;
   if (TC) { Tm = qdiv(Pm1.p, Pm1.r); T0 = qdiv(P0.p, P0.r); T1 = qdiv(P1.p, P1.r); }
@@ -331,14 +365,17 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
     real T2 = 0;
     if (TC) T2 = qdiv(P2.p, P2.r);
     Prim<real> H{}, Hm{}, Hp{};
-    if (hown) {
+    if (VISC && hown) {
+      H = Prim<real>{fz[0], fz[1], fz[2], fz[3], fz[4]};
+      Hm.u = sq[1][hr][hc]; Hm.v = sq[2][hr][hc]; Hm.w = sq[3][hr][hc];  // plane k-1
+      Hp = halo_prim(k + 1, cz);
+    } else if (hown) {
       bool hfz;
       map_coord(k, P.Nz, P.bcz, hfz);
       H = cons_to_prim(cz[0], cz[1], cz[2], cz[3], cz[4], gm1);
       if (hfx) H.u = -H.u;
       if (hfy) H.v = -H.v;
       if (hfz) H.w = -H.w;
-      if (VISC) { Hm = load_halo(k - 1); Hp = load_halo(k + 1); }
     }
 
     if (NBUF == 1) __syncthreads();  // readers of the previous plane are done
@@ -357,7 +394,12 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
         svm[0][hr][hc] = Hm.u; svm[1][hr][hc] = Hm.v; svm[2][hr][hc] = Hm.w;
         svp[0][hr][hc] = Hp.u; svp[1][hr][hc] = Hp.v; svp[2][hr][hc] = Hp.w;
       }
-      if (k + 1 < k1) fetch_halo(k + 1, cz);  // consumed next plane
+      if (VISC) {
+        fz[0] = Hp.r; fz[1] = Hp.u; fz[2] = Hp.v; fz[3] = Hp.w; fz[4] = Hp.p;
+        if (k + 1 < k1) fetch_halo_any(k + 2, cz);  // consumed next plane
+      } else if (k + 1 < k1) {
+        fetch_halo(k + 1, cz);  // consumed next plane
+      }
     }
     __syncthreads();
 
