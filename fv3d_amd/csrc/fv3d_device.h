@@ -88,6 +88,68 @@ __device__ __forceinline__ double sqrt_f64(double x) {
   return x > 0.0 ? s : (x == 0.0 ? 0.0 : __builtin_nan(""));
 }
 
+// ---------------------------------------------------------------------------
+// FP64 quotients and square roots without the range-scaling steps (the
+// FAST template switch below).  The compiler's correctly rounded FP64
+// division is v_div_scale (twice), v_rcp, four FMAs refining the
+// reciprocal, a product, an FMA for the remainder, v_div_fmas and
+// v_div_fixup.  v_div_scale returns its operand unchanged (and v_div_fmas
+// is then a plain FMA) unless the numerator is below about 2^-960 in
+// magnitude, the denominator is denormal or the quotient is denormal; zero,
+// infinite and NaN operands are settled by v_div_fixup, which both forms
+// end with.  Outside those cases the sequence without v_div_scale gives the
+// same bits (measured on MI355X over 2.7e8 random pairs per exponent range
+// and every pair of special values), and quotients with one denominator can
+// share its refined reciprocal.  The square root likewise drops its scaling
+// for positive finite arguments of at least 2^-767 (the zero and infinity
+// fix goes too).  The
+// caller guarantees the operand ranges (the stage kernel checks every cell
+// it loads, cell_in_fast_range).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double rcp_refined(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  double e = __builtin_fma(-d, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-d, r, 1.0);
+  return __builtin_fma(r, e, r);
+}
+__device__ __forceinline__ double div_refined(double n, double d, double r) {
+  const double q = n * r;
+  const double rem = __builtin_fma(-d, q, n);
+  return __builtin_amdgcn_div_fixup(__builtin_fma(rem, r, q), d, n);
+}
+__device__ __forceinline__ double sqrt_unscaled(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = y * 0.5;
+  const double e = __builtin_fma(-h, g, 0.5);
+  g = __builtin_fma(g, e, g);
+  h = __builtin_fma(h, e, h);
+  double t = __builtin_fma(-g, g, x);
+  g = __builtin_fma(t, h, g);
+  t = __builtin_fma(-g, g, x);
+  return __builtin_fma(t, h, g);
+}
+// quotient, shared reciprocal and square root of the stage arithmetic:
+// FAST (FP64 only) takes the forms above, otherwise qdiv and sqrt
+template <typename real, bool FAST>
+struct Ar {
+  static __device__ __forceinline__ real rcp(real b) { return b; }
+  static __device__ __forceinline__ real divr(real a, real b, real) { return qdiv(a, b); }
+  static __device__ __forceinline__ real div(real a, real b) { return qdiv(a, b); }
+  static __device__ __forceinline__ real sqr(real x) { return sqrt(x); }
+};
+template <>
+struct Ar<double, true> {
+  static __device__ __forceinline__ double rcp(double b) { return rcp_refined(b); }
+  static __device__ __forceinline__ double divr(double a, double b, double r) {
+    return div_refined(a, b, r);
+  }
+  static __device__ __forceinline__ double div(double a, double b) {
+    return div_refined(a, b, rcp_refined(b));
+  }
+  static __device__ __forceinline__ double sqr(double x) { return sqrt_unscaled(x); }
+};
+
 // t / h for a cell width h (1/h = ih): the product when h is a power of two
 // (the same bits) and in FP32, else the quotient
 template <typename real>
@@ -122,9 +184,31 @@ __device__ __forceinline__ void prim_to_cons(const Prim<real>& q, real gm1, real
 }
 
 // States.h:47-50
-template <typename real>
+template <typename real, bool FAST = false>
 __device__ __forceinline__ real sound_speed(real r, real p, real gamma) {
-  return sqrt(qdiv(p * gamma, r));
+  using A = Ar<real, FAST>;
+  return A::sqr(A::div(p * gamma, r));
+}
+
+// FAST operand ranges of one cell (FP64): rho and p in [2^-40, 2^41), each
+// momentum zero or of magnitude in [2^-40, 2^41).  PLM-minmod face states
+// lie between a cell's value and the mean with its neighbour, so face
+// densities and pressures stay within a factor 2 of that range and face
+// velocities are zero or of magnitude in [2^-134, 2^82].  Every sum,
+// difference and product of HLL is then zero or at least the ulp of its
+// smallest operand, which bounds its numerators (zero or above 2^-760,
+// against the 2^-960 where v_div_scale starts to scale), its quotients
+// (normal) and the sound speed's argument (positive, finite and above
+// 2^-767), so the FAST forms change no bit.  A positive value in the range
+// has its high word in [0x3D700000, 0x42800000).
+__device__ __forceinline__ bool cell_in_fast_range(double U0, double U1, double U2, double U3,
+                                                   double p) {
+  auto hi = [](double x) { return (unsigned)(__double_as_longlong(x) >> 32); };
+  auto pos_ok = [&](double x) { return hi(x) - 0x3D700000u < 0x42800000u - 0x3D700000u; };
+  auto mom_ok = [&](double x) {
+    return ((hi(x) & 0x7FFFFFFFu) - 0x3D700000u < 0x42800000u - 0x3D700000u) || x == 0.0;
+  };
+  return pos_ok(U0) && pos_ok(p) && mom_ok(U1) && mom_ok(U2) && mom_ok(U3);
 }
 
 // Update.h:73-80: minmod(a, b) = a*b<0 ? 0 : (|a|<|b| ? a : b).  The product
@@ -198,14 +282,16 @@ __device__ __forceinline__ void hllc(const Prim<real> L, const Prim<real> R, rea
 
 // RiemannSolvers.h:5-36 with computeFlux (States.h:113-127) and primToCons
 // (States.h:20-31).  p/(gamma0-1) appears in both and is computed once.
-template <typename real>
+template <typename real, bool FAST = false>
 __device__ __forceinline__ void hll(const Prim<real> L, const Prim<real> R, real gamma, real gm1,
                                     real F[5], real& pout) {
-  const real aL = sound_speed(L.r, L.p, gamma);
-  const real aR = sound_speed(R.r, R.p, gamma);
+  using A = Ar<real, FAST>;
+  const real aL = sound_speed<real, FAST>(L.r, L.p, gamma);
+  const real aR = sound_speed<real, FAST>(R.r, R.p, gamma);
   const real SL = fmin(L.u - aL, R.u - aR);
   const real SR = fmax(L.u + aL, R.u + aR);
-  const real peL = qdiv(L.p, gm1), peR = qdiv(R.p, gm1);
+  const real ig = A::rcp(gm1);
+  const real peL = A::divr(L.p, gm1, ig), peR = A::divr(R.p, gm1, ig);
   const real EkL = (real)0.5 * L.r * (L.u * L.u + L.v * L.v + L.w * L.w);
   const real EkR = (real)0.5 * R.r * (R.u * R.u + R.v * R.v + R.w * R.w);
   const real EfL = (peL + EkL), EfR = (peR + EkR);
@@ -225,21 +311,21 @@ __device__ __forceinline__ void hll(const Prim<real> L, const Prim<real> R, real
   } else {
     real UL[5], UR[5];
     UL[0] = L.r; UL[1] = L.r * L.u; UL[2] = L.r * L.v; UL[3] = L.r * L.w;
-    UL[4] = (qdiv((real)0.5 * (UL[1] * UL[1] + UL[2] * UL[2] + UL[3] * UL[3]), L.r) + peL);
+    UL[4] = (A::div((real)0.5 * (UL[1] * UL[1] + UL[2] * UL[2] + UL[3] * UL[3]), L.r) + peL);
     UR[0] = R.r; UR[1] = R.r * R.u; UR[2] = R.r * R.v; UR[3] = R.r * R.w;
-    UR[4] = (qdiv((real)0.5 * (UR[1] * UR[1] + UR[2] * UR[2] + UR[3] * UR[3]), R.r) + peR);
+    UR[4] = (A::div((real)0.5 * (UR[1] * UR[1] + UR[2] * UR[2] + UR[3] * UR[3]), R.r) + peR);
     pout = (real)0.5 * (L.p + R.p);
-    const real SLSR = SL * SR, dS = SR - SL;
+    const real SLSR = SL * SR, dS = SR - SL, iS = A::rcp(dS);
 #pragma unroll
-    for (int n = 0; n < 5; ++n) F[n] = qdiv(FL[n] * SR - FR[n] * SL + (UR[n] - UL[n]) * SLSR, dS);
+    for (int n = 0; n < 5; ++n) F[n] = A::divr(FL[n] * SR - FR[n] * SL + (UR[n] - UL[n]) * SLSR, dS, iS);
   }
 }
 
-template <typename real, int RS>
+template <typename real, int RS, bool FAST = false>
 __device__ __forceinline__ void riemann(const Prim<real> L, const Prim<real> R, real gamma,
                                         real gm1, real entho, real F[5], real& pout) {
-  if (RS == FV3D_HLL) hll(L, R, gamma, gm1, F, pout);
-  else hllc(L, R, gamma, entho, F, pout);
+  if (RS == FV3D_HLL) hll<real, FAST>(L, R, gamma, gm1, F, pout);
+  else hllc(L, R, gamma, entho, F, pout);  // FAST applies to HLL only
 }
 
 // swap_component (States.h:103-111) on a state in the face frame

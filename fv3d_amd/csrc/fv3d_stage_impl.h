@@ -28,6 +28,7 @@
 // 96-cell-wide boxes and re-converts fewer halo cells per interior cell
 // (36 x 20 window cells for 32 x 16, against 68 x 12 for 64 x 8).
 #pragma once
+#include <type_traits>
 #include "fv3d_device.h"
 #include "fv3d_kernels.h"
 #include "fv3d_physics.h"
@@ -43,13 +44,26 @@ namespace fv3d_hip {
 // kernel (the compensated state alternates between two arrays).
 enum { STAGE_PLAIN = 0, STAGE_INC1 = 1, STAGE_INC2 = 2 };
 
-template <typename real>
+// consToPrim of a loaded cell; CHECK (the FP64 kernels with FAST faces)
+// also tells whether the cell is outside the FAST range (bad).  The
+// conversion keeps its correctly rounded quotients: a FAST form would need
+// a per-lane fallback branch, which cost more than it saved.
+template <typename real, bool CHECK>
+__device__ __forceinline__ Prim<real> cons_to_prim_checked(real u0, real u1, real u2, real u3,
+                                                           real u4, real gm1, bool& bad) {
+  const Prim<real> q = cons_to_prim(u0, u1, u2, u3, u4, gm1);
+  bad = CHECK && sizeof(real) == 8 &&
+        !cell_in_fast_range((double)u0, (double)u1, (double)u2, (double)u3, (double)q.p);
+  return q;
+}
+
+template <typename real, bool CHECK = false>
 __device__ __forceinline__ Prim<real> load_prim(const real* __restrict__ U, long long base,
                                                 long long sf, real gm1, bool fx, bool fy,
-                                                bool fz) {
+                                                bool fz, bool& bad) {
   const real u0 = U[base], u1 = U[base + sf], u2 = U[base + 2 * sf], u3 = U[base + 3 * sf],
              u4 = U[base + 4 * sf];
-  Prim<real> q = cons_to_prim(u0, u1, u2, u3, u4, gm1);
+  Prim<real> q = cons_to_prim_checked<real, CHECK>(u0, u1, u2, u3, u4, gm1, bad);
   // reflecting ghosts: BoundaryConditions.h:45-50 negates the normal velocity
   if (fx) q.u = -q.u;
   if (fy) q.v = -q.v;
@@ -95,7 +109,7 @@ __device__ __forceinline__ Prim<real> minus_face(const Prim<real>& q, const Prim
 
 // Flux through the face between cells b and c (a before b, d after c) along
 // DIR, in the lab frame.
-template <typename real, int DIR, int RECON, int RS>
+template <typename real, int DIR, int RECON, int RS, bool FAST = false>
 __device__ __forceinline__ void face_flux4(const Prim<real>& a, const Prim<real>& b,
                                            const Prim<real>& c, const Prim<real>& d,
                                            const DevParams& P, real F[5], real& pout) {
@@ -113,15 +127,15 @@ __device__ __forceinline__ void face_flux4(const Prim<real>& a, const Prim<real>
       qR.p = c.p - c.r * (real)P.g * (real)P.dz * (real)0.5;
     }
   }
-  riemann<real, RS>(rotate<real, DIR>(qL), rotate<real, DIR>(qR), (real)P.gamma, (real)P.gm1,
-                    (real)P.entho, F, pout);
+  riemann<real, RS, FAST>(rotate<real, DIR>(qL), rotate<real, DIR>(qR), (real)P.gamma,
+                          (real)P.gm1, (real)P.entho, F, pout);
   unrotate_flux<real, DIR>(F);
 }
 
 // face_flux4 along x or along y, chosen per lane (isy): on every lane the
 // same operations on the same values as face_flux4<real, FV3D_IX or
 // FV3D_IY, ...>; the rotation into the face frame and back is a select.
-template <typename real, int RECON, int RS>
+template <typename real, int RECON, int RS, bool FAST = false>
 __device__ __forceinline__ void face_flux_xy(const Prim<real>& a, const Prim<real>& b,
                                              const Prim<real>& c, const Prim<real>& d, bool isy,
                                              const DevParams& P, real F[5], real& pout) {
@@ -138,7 +152,7 @@ __device__ __forceinline__ void face_flux_xy(const Prim<real>& a, const Prim<rea
   auto rot = [&](const Prim<real>& q) {
     return Prim<real>{q.r, isy ? q.v : q.u, isy ? q.u : q.v, q.w, q.p};
   };
-  riemann<real, RS>(rot(qL), rot(qR), (real)P.gamma, (real)P.gm1, (real)P.entho, F, pout);
+  riemann<real, RS, FAST>(rot(qL), rot(qR), (real)P.gamma, (real)P.gm1, (real)P.entho, F, pout);
   const real f1 = F[1], f2 = F[2];
   F[1] = isy ? f2 : f1;
   F[2] = isy ? f1 : f2;
@@ -185,6 +199,14 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
   // 3x3x3 stencil, whose shared LDS reads cost less than re-reading each
   // face's stencil (measured 8% faster on the stratified box)
   constexpr bool VFACE = VISC && sizeof(real) == 4;
+  // FP64 PLM-HLL faces (not C91, whose ghost planes are not range-checked)
+  // take the FAST quotients and square roots in every plane where all the
+  // cells they read are in the FAST range (cell_in_fast_range); other
+  // planes, HLLC and the FP32 kernels keep their plain forms.  (HLLC in the
+  // FAST form measured no faster on the 256^3 blast: its quotients share
+  // fewer denominators than HLL's star flux, and the range test costs what
+  // the rest saves.)
+  constexpr bool FASTK = sizeof(real) == 8 && RECON == FV3D_PLM && RS == FV3D_HLL && !C91Z;
   __shared__ real sqb[NBUF][5][NR][NC];
   __shared__ real sTb[NBUF][TC ? NR : 1][TC ? NC : 1];
   __shared__ real svm[VISC ? 3 : 1][VISC ? NR : 1][VISC ? NC : 1];
@@ -196,6 +218,7 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
   __shared__ real svx[VFACE ? 4 : 1][VFACE ? TY : 1][VFACE ? WX + 1 : 1];
   __shared__ real svy[VFACE ? 4 : 1][VFACE ? TY + 1 : 1][VFACE ? WX : 1];
   __shared__ double sred[NW];
+  __shared__ int sfl[2][NW];  // FASTK: per wave, per plane parity
 
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -260,17 +283,18 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
     if (gy) q.v = -q.v;
     return q;
   };
-  auto load_plane = [&](int kk) -> Prim<real> {
-    if (C91Z && (kk < 0 || kk >= P.Nz)) return c91_ghost(kk, sxy, fx, fy);
+  // bad: the cell is outside the FAST range (always false without FASTK)
+  auto load_plane = [&](int kk, bool& bad) -> Prim<real> {
+    if (C91Z && (kk < 0 || kk >= P.Nz)) { bad = true; return c91_ghost(kk, sxy, fx, fy); }
     bool fz;
     const int sk = map_coord(kk, P.Nz, P.bcz, fz);
-    return load_prim<real>(Uin, sxy + (long long)sk * P.sz, P.sf, gm1, fx, fy, fz);
+    return load_prim<real, FASTK>(Uin, sxy + (long long)sk * P.sz, P.sf, gm1, fx, fy, fz, bad);
   };
-  auto load_halo = [&](int kk) -> Prim<real> {
-    if (C91Z && (kk < 0 || kk >= P.Nz)) return c91_ghost(kk, hxy, hfx, hfy);
+  auto load_halo = [&](int kk, bool& bad) -> Prim<real> {
+    if (C91Z && (kk < 0 || kk >= P.Nz)) { bad = true; return c91_ghost(kk, hxy, hfx, hfy); }
     bool fz;
     const int sk = map_coord(kk, P.Nz, P.bcz, fz);
-    return load_prim<real>(Uin, hxy + (long long)sk * P.sz, P.sf, gm1, hfx, hfy, fz);
+    return load_prim<real, FASTK>(Uin, hxy + (long long)sk * P.sz, P.sf, gm1, hfx, hfy, fz, bad);
   };
   // raw conserved values of the halo cell at plane kk, converted later
   auto fetch_halo = [&](int kk, real h[5]) {
@@ -291,11 +315,11 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
     }
     fetch_halo(kk, h);
   };
-  auto halo_prim = [&](int kk, const real h[5]) -> Prim<real> {
-    if (C91Z && (kk < 0 || kk >= P.Nz)) return Prim<real>{h[0], h[1], h[2], h[3], h[4]};
+  auto halo_prim = [&](int kk, const real h[5], bool& bad) -> Prim<real> {
+    if (C91Z && (kk < 0 || kk >= P.Nz)) { bad = true; return Prim<real>{h[0], h[1], h[2], h[3], h[4]}; }
     bool hfz;
     map_coord(kk, P.Nz, P.bcz, hfz);
-    Prim<real> q = cons_to_prim(h[0], h[1], h[2], h[3], h[4], gm1);
+    Prim<real> q = cons_to_prim_checked<real, FASTK>(h[0], h[1], h[2], h[3], h[4], gm1, bad);
     if (hfx) q.u = -q.u;
     if (hfy) q.v = -q.v;
     if (hfz) q.w = -q.w;
@@ -311,11 +335,17 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
   real cz[5] = {0, 0, 0, 0, 0}, fz[5] = {0, 0, 0, 0, 0}, pz = 0;
 
   // ---- warm-up: planes k0-2 .. k0+1, z slope at k0 and the z face k0-1/2
-  const Prim<real> Pm1 = load_plane(k0 - 1);
-  Prim<real> P0 = load_plane(k0);
-  Prim<real> P1 = load_plane(k0 + 1);
+  // (the z face always in the plain form).  bb: the lane's cell is outside
+  // the FAST range in planes k-1, k, k+1, k+2 (bits 3..0); hb1: so is its
+  // halo cell in plane k+1 (with viscosity, converted one plane ahead).
+  bool bm1, b0, b1, hb1 = false;
+  const Prim<real> Pm1 = load_plane(k0 - 1, bm1);
+  Prim<real> P0 = load_plane(k0, b0);
+  Prim<real> P1 = load_plane(k0 + 1, b1);
+  unsigned bb = (bm1 ? 4u : 0u) | (b0 ? 2u : 0u) | (b1 ? 1u : 0u);
   if (interior) {
-    const Prim<real> Pm2 = load_plane(k0 - 2);
+    bool bm2;
+    const Prim<real> Pm2 = load_plane(k0 - 2, bm2);
     if (RECON == FV3D_PLM) {
       const Prim<real> s = slope3<real, true>(Pm1, P0, P1);
       cz[0] = s.r; cz[1] = s.u; cz[2] = s.v; cz[3] = s.w; cz[4] = s.p;
@@ -325,9 +355,10 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
     if (VISC) {
       // halo column ring: fz = primitives of plane k, cz = raw values of
       // plane k+1, and the cell's LDS slot holds the velocities of plane k-1
-      const Prim<real> A = load_halo(k0);
+      const Prim<real> A = load_halo(k0, hb1);
       fz[0] = A.r; fz[1] = A.u; fz[2] = A.v; fz[3] = A.w; fz[4] = A.p;
-      const Prim<real> B = load_halo(k0 - 1);
+      bool bbm;
+      const Prim<real> B = load_halo(k0 - 1, bbm);
       sqb[0][1][hr][hc] = B.u; sqb[0][2][hr][hc] = B.v; sqb[0][3][hr][hc] = B.w;
       fetch_halo_any(k0 + 1, cz);
     } else {
@@ -361,18 +392,46 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
         return svp[v - 1][rr][cc];
       };
     };
-    const Prim<real> P2 = load_plane(k + 2);
+    // plane k+2: loaded here, converted in the face phase (its first use),
+    // so its loads are in flight across the publish
+    real raw2[5];
+    bool flip2 = false;
+    const bool ghost2 = C91Z && (k + 2 >= P.Nz);
+    if (ghost2) {
+      const Prim<real> g = c91_ghost(k + 2, sxy, fx, fy);
+      raw2[0] = g.r; raw2[1] = g.u; raw2[2] = g.v; raw2[3] = g.w; raw2[4] = g.p;
+    } else {
+      const int sk = map_coord(k + 2, P.Nz, P.bcz, flip2);
+      const long long base = sxy + (long long)sk * P.sz;
+#pragma unroll
+      for (int n = 0; n < 5; ++n) raw2[n] = Uin[base + n * P.sf];
+    }
+    Prim<real> P2{};
     real T2 = 0;
-    if (TC) T2 = qdiv(P2.p, P2.r);
+    auto convert_p2 = [&](bool& bad2) {
+      if (ghost2) {
+        bad2 = true;
+        P2 = Prim<real>{raw2[0], raw2[1], raw2[2], raw2[3], raw2[4]};
+      } else {
+        P2 = cons_to_prim_checked<real, FASTK>(raw2[0], raw2[1], raw2[2], raw2[3], raw2[4], gm1,
+                                               bad2);
+        if (fx) P2.u = -P2.u;
+        if (fy) P2.v = -P2.v;
+        if (flip2) P2.w = -P2.w;
+      }
+      if (TC) T2 = qdiv(P2.p, P2.r);
+    };
     Prim<real> H{}, Hm{}, Hp{};
+    bool hbad = false;  // the halo cell of plane k
     if (VISC && hown) {
       H = Prim<real>{fz[0], fz[1], fz[2], fz[3], fz[4]};
+      hbad = hb1;
       Hm.u = sq[1][hr][hc]; Hm.v = sq[2][hr][hc]; Hm.w = sq[3][hr][hc];  // plane k-1
-      Hp = halo_prim(k + 1, cz);
+      Hp = halo_prim(k + 1, cz, hb1);
     } else if (hown) {
       bool hfz;
       map_coord(k, P.Nz, P.bcz, hfz);
-      H = cons_to_prim(cz[0], cz[1], cz[2], cz[3], cz[4], gm1);
+      H = cons_to_prim_checked<real, FASTK>(cz[0], cz[1], cz[2], cz[3], cz[4], gm1, hbad);
       if (hfx) H.u = -H.u;
       if (hfy) H.v = -H.v;
       if (hfz) H.w = -H.w;
@@ -401,7 +460,20 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
         fetch_halo(k + 1, cz);  // consumed next plane
       }
     }
+    // FASTK: one flag per wave, a cell of plane k (its x and y faces) is out
+    // of range; bb holds planes k-1, k, k+1 in bits 2..0 here
+    if (FASTK) {
+      const bool wb = __ballot((bb & 2u) != 0u || hbad) != 0ull;  // plane k
+      if (lane == 0) sfl[k & 1][wave] = wb ? 1 : 0;
+    }
     __syncthreads();
+    bool fast = false;
+    if (FASTK) {
+      int any = 0;
+#pragma unroll
+      for (int w = 0; w < NW; ++w) any |= sfl[k & 1][w];
+      fast = any == 0;
+    }
 
     // the update's base state (and U^n of the RK2 combine) are fetched here,
     // so their latency hides behind the face solves
@@ -432,72 +504,97 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
     if (store && !LATE_U) fetch_u();
     if (store && !LATE_RK) fetch_rk();
 
-    // ---- x faces: face between window columns c-1 and c for every interior row
-    if (interior) {
-      real F[5], po;
-      face_flux4<real, FV3D_IX, RECON, RS>(lds_prim(r, c - 2), lds_prim(r, c - 1), P0,
-                                           lds_prim(r, c + 1), P, F, po);
-#pragma unroll
-      for (int n = 0; n < 5; ++n) sfx[n][r - 2][xl] = F[n];
-    } else if (xedge) {
-      // the right face of the last interior column for row lane+2
-      real F[5], po;
-      face_flux4<real, FV3D_IX, RECON, RS>(lds_prim(lane + 2, WX), lds_prim(lane + 2, WX + 1),
-                                           lds_prim(lane + 2, WX + 2),
-                                           lds_prim(lane + 2, WX + 3), P, F, po);
-#pragma unroll
-      for (int n = 0; n < 5; ++n) sfx[n][lane][WX] = F[n];
-    }
-    if (mwave) {
-      // top y faces (lanes 0-31: row TY+2) and edge x faces (lanes 32..:
-      // rows 2..TY+1) in one face evaluation
-      const bool isy = lane < 32;
-      const int er = lane - 32 + 2;
-      if (isy || er < TY + 2) {
-        const int r0 = isy ? r - 2 : er, c0 = isy ? c : WX;
-        const int dr = isy ? 1 : 0, dc = isy ? 0 : 1;
-        real F[5], po;
-        face_flux_xy<real, RECON, RS>(lds_prim(r0, c0), lds_prim(r0 + dr, c0 + dc),
-                                      lds_prim(r0 + 2 * dr, c0 + 2 * dc),
-                                      lds_prim(r0 + 3 * dr, c0 + 3 * dc), isy, P, F, po);
-        if (isy) {
-#pragma unroll
-          for (int n = 0; n < 5; ++n) sfy[n][r - 2][xl] = F[n];
-        } else {
-#pragma unroll
-          for (int n = 0; n < 5; ++n) sfx[n][er - 2][WX] = F[n];
-        }
-      }
-    }
-    // ---- y faces: face between window rows r-1 and r
-    if (yworker && !mwave) {
-      real F[5], po;
-      face_flux4<real, FV3D_IY, RECON, RS>(lds_prim(r - 2, c), lds_prim(r - 1, c), P0,
-                                           lds_prim(r + 1, c), P, F, po);
-#pragma unroll
-      for (int n = 0; n < 5; ++n) sfy[n][r - 2][xl] = F[n];
-    }
-    // ---- z face k+1/2 from registers
     real Fzh[5] = {0, 0, 0, 0, 0}, pouth = 0;
     Prim<real> sK1{};
-    if (interior) {
-      Prim<real> qL, qR;
-      if (RECON == FV3D_PLM) {
-        const Prim<real> sK{cz[0], cz[1], cz[2], cz[3], cz[4]};
-        sK1 = slope3<real, true>(P0, P1, P2);
-        qL = plus_face(P0, sK);
-        qR = minus_face(P1, sK1);
-      } else {
-        qL = P0;
-        qR = P1;
-        if (pcm_wb) {
-          qL.p = P0.p + P0.r * (real)P.g * (real)P.dz * (real)0.5;
-          qR.p = P1.p - P1.r * (real)P.g * (real)P.dz * (real)0.5;
+    // The x and y face solves, in the FAST form (FASTK, every cell of plane k
+    // in range: uniform over the workgroup) or the plain one.
+    auto face_xy = [&](auto fast_tag) {
+      constexpr bool FA = decltype(fast_tag)::value;
+      // ---- x faces: face between window columns c-1 and c for every interior row
+      if (interior) {
+        real F[5], po;
+        face_flux4<real, FV3D_IX, RECON, RS, FA>(lds_prim(r, c - 2), lds_prim(r, c - 1), P0,
+                                             lds_prim(r, c + 1), P, F, po);
+#pragma unroll
+        for (int n = 0; n < 5; ++n) sfx[n][r - 2][xl] = F[n];
+      } else if (xedge) {
+        // the right face of the last interior column for row lane+2
+        real F[5], po;
+        face_flux4<real, FV3D_IX, RECON, RS, FA>(lds_prim(lane + 2, WX), lds_prim(lane + 2, WX + 1),
+                                             lds_prim(lane + 2, WX + 2),
+                                             lds_prim(lane + 2, WX + 3), P, F, po);
+#pragma unroll
+        for (int n = 0; n < 5; ++n) sfx[n][lane][WX] = F[n];
+      }
+      if (mwave) {
+        // top y faces (lanes 0-31: row TY+2) and edge x faces (lanes 32..:
+        // rows 2..TY+1) in one face evaluation
+        const bool isy = lane < 32;
+        const int er = lane - 32 + 2;
+        if (isy || er < TY + 2) {
+          const int r0 = isy ? r - 2 : er, c0 = isy ? c : WX;
+          const int dr = isy ? 1 : 0, dc = isy ? 0 : 1;
+          real F[5], po;
+          face_flux_xy<real, RECON, RS, FA>(lds_prim(r0, c0), lds_prim(r0 + dr, c0 + dc),
+                                        lds_prim(r0 + 2 * dr, c0 + 2 * dc),
+                                        lds_prim(r0 + 3 * dr, c0 + 3 * dc), isy, P, F, po);
+          if (isy) {
+#pragma unroll
+            for (int n = 0; n < 5; ++n) sfy[n][r - 2][xl] = F[n];
+          } else {
+#pragma unroll
+            for (int n = 0; n < 5; ++n) sfx[n][er - 2][WX] = F[n];
+          }
         }
       }
-      riemann<real, RS>(rotate<real, FV3D_IZ>(qL), rotate<real, FV3D_IZ>(qR), (real)P.gamma, gm1,
-                        (real)P.entho, Fzh, pouth);
-      unrotate_flux<real, FV3D_IZ>(Fzh);
+      // ---- y faces: face between window rows r-1 and r
+      if (yworker && !mwave) {
+        real F[5], po;
+        face_flux4<real, FV3D_IY, RECON, RS, FA>(lds_prim(r - 2, c), lds_prim(r - 1, c), P0,
+                                             lds_prim(r + 1, c), P, F, po);
+#pragma unroll
+        for (int n = 0; n < 5; ++n) sfy[n][r - 2][xl] = F[n];
+      }
+    };
+    if constexpr (FASTK) {
+      if (fast) face_xy(std::true_type{});
+      else face_xy(std::false_type{});
+    } else {
+      face_xy(std::false_type{});
+    }
+    // plane k+2, then the z face k+1/2 in the FAST form when the wave's
+    // cells of planes k-1 .. k+2 are all in range (uniform over the wave)
+    bool bad2;
+    convert_p2(bad2);
+    const unsigned bb4 = ((bb << 1) | (bad2 ? 1u : 0u)) & 15u;
+    auto face_z = [&](auto fast_tag) {
+      constexpr bool FZ = decltype(fast_tag)::value;
+      // ---- z face k+1/2 from registers
+      if (interior) {
+        Prim<real> qL, qR;
+        if (RECON == FV3D_PLM) {
+          const Prim<real> sK{cz[0], cz[1], cz[2], cz[3], cz[4]};
+          sK1 = slope3<real, true>(P0, P1, P2);
+          qL = plus_face(P0, sK);
+          qR = minus_face(P1, sK1);
+        } else {
+          qL = P0;
+          qR = P1;
+          if (pcm_wb) {
+            qL.p = P0.p + P0.r * (real)P.g * (real)P.dz * (real)0.5;
+            qR.p = P1.p - P1.r * (real)P.g * (real)P.dz * (real)0.5;
+          }
+        }
+        riemann<real, RS, FZ>(rotate<real, FV3D_IZ>(qL), rotate<real, FV3D_IZ>(qR), (real)P.gamma, gm1,
+                          (real)P.entho, Fzh, pouth);
+        unrotate_flux<real, FV3D_IZ>(Fzh);
+      }
+    };
+    if constexpr (FASTK) {
+      if (__ballot(bb4 != 0u) == 0ull) face_z(std::true_type{});
+      else face_z(std::false_type{});
+    } else {
+      face_z(std::false_type{});
     }
     if (VFACE) {
       // viscous x face c-1/2 of the interior rows (side 1 of cell c) and the
@@ -667,6 +764,7 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
     if (TC) { Tm = T0; T0 = T1; T1 = T2; }
     P0 = P1;
     P1 = P2;
+    bb = bb4 & 7u;
     if (interior) {
       if (RECON == FV3D_PLM) { cz[0] = sK1.r; cz[1] = sK1.u; cz[2] = sK1.v; cz[3] = sK1.w; cz[4] = sK1.p; }
 #pragma unroll
@@ -747,8 +845,9 @@ k_c91_ghosts(const real* __restrict__ U, real* g, DevParams P) {
   const int kref = kk < 0 ? 0 : P.Nz - 1;
   const int ksym = kk < 0 ? -kk - 1 : 2 * P.Nz - kk - 1;
   const real gm1 = (real)P.gm1;
-  const Prim<real> qr = load_prim<real>(U, xy + (long long)kref * P.sz, P.sf, gm1, false, false, false);
-  const Prim<real> qs = load_prim<real>(U, xy + (long long)ksym * P.sz, P.sf, gm1, false, false, false);
+  bool nb;
+  const Prim<real> qr = load_prim<real>(U, xy + (long long)kref * P.sz, P.sf, gm1, false, false, false, nb);
+  const Prim<real> qs = load_prim<real>(U, xy + (long long)ksym * P.sz, P.sf, gm1, false, false, false, nb);
   const real z = (real)P.zmin + ((real)kk + (real)0.5) * (real)P.dz;
   const real grav = P.gravity ? (real)1 : (real)0;
   real res[5];

@@ -177,6 +177,47 @@ def test_hll_fixed_temperature_visc_gravity(gpu_available):
     _compare_runs(p, Q0, max_steps=8)
 
 
+def _fast_range_state(p, case, seed):
+    """Primitive states for the HLL FAST-quotient cases.  "wide": every
+    cell in the range the kernels test (rho and p in [2^-40, 2^41), momenta
+    zero or of magnitude in [2^-40, 2^41)), spanning decades, a fifth of the
+    velocities exactly zero.  "edges": a smooth state with planes whose
+    cells sit just outside that range (a density of 2^-41, a pressure of
+    2^41, a momentum of 1e-13), so those planes and their neighbours take the
+    correctly rounded forms."""
+    rng = np.random.default_rng(seed)
+    Q = smooth_random(p, seed, amp_u=0.3)
+    d = dom(p, Q)
+    if case == "wide":
+        shape = d.shape[:3]
+        d[..., 0] = 2.0 ** rng.uniform(-12, 12, shape)
+        d[..., 4] = d[..., 0] * 2.0 ** rng.uniform(-6, 6, shape)
+        for v in (1, 2, 3):
+            speed = np.sqrt(d[..., 4] / d[..., 0]) * 2.0 ** rng.uniform(-14, 1, shape)
+            d[..., v] = rng.choice([-1.0, 1.0], shape) * speed * (rng.random(shape) > 0.2)
+    else:
+        d[3, 2:5, 4:9, 0] = 2.0 ** -41
+        d[3, 2:5, 4:9, 4] = 2.0 ** -40
+        d[7, :, 5, 4] = 2.0 ** 41
+        d[11, 6, :, 1] = 1e-13
+        d[12, 1, 2, 2] = 0.0
+    return Q
+
+
+@pytest.mark.usefixtures("nzc")
+@pytest.mark.parametrize("case", ["wide", "edges"])
+def test_hll_fast_quotients_bitwise(gpu_available, case):
+    """FP64 PLM-HLL faces take their quotients and square roots without the
+    range-scaling steps on planes whose cells are all in a tested range
+    (cell_in_fast_range), and the correctly rounded forms elsewhere; both
+    give the reference's bits, including planes that switch between them."""
+    p = Params.defaults().replace(Nx=24, Ny=10, Nz=16, boundary_x=F.BC_PERIODIC,
+                                  boundary_y=F.BC_REFLECTING, boundary_z=F.BC_ABSORBING,
+                                  reconstruction=F.PLM, riemann_solver=F.HLL,
+                                  time_stepping=F.TS_RK2, CFL=0.4, tend=10.0)
+    _compare_runs(p, _fast_range_state(p, case, 21), max_steps=5)
+
+
 @pytest.mark.usefixtures("nzc")
 def test_fixed_gradient_and_absorbing(gpu_available):
     p = Params.defaults().replace(Nx=20, Ny=12, Nz=10, boundary_x=F.BC_ABSORBING, boundary_y=F.BC_REFLECTING,
