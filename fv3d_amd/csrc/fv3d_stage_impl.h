@@ -631,14 +631,39 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
       real u[5];
 #pragma unroll
       for (int n = 0; n < 5; ++n) u[n] = ub[n];
+      // the face fluxes' differences.  The empty asm statements take all 20
+      // flux values as operands, so their LDS reads are issued together: left
+      // to itself, the FP64 kernels' register allocation sent them out one
+      // pair at a time, each waiting for the last (ten LDS round trips per
+      // interior wave and plane)
+      real dx[5], dy[5];
+      {
+        real a[5], b[5], e[5], f[5];
+#pragma unroll
+        for (int n = 0; n < 5; ++n) {
+          a[n] = sfx[n][r - 2][xl];
+          b[n] = sfx[n][r - 2][xl + 1];
+          e[n] = sfy[n][r - 2][xl];
+          f[n] = sfy[n][r - 1][xl];
+        }
+        asm volatile("" ::"v"(a[0]), "v"(a[1]), "v"(a[2]), "v"(a[3]), "v"(a[4]), "v"(b[0]), "v"(b[1]),
+                     "v"(b[2]), "v"(b[3]), "v"(b[4]));
+        asm volatile("" ::"v"(e[0]), "v"(e[1]), "v"(e[2]), "v"(e[3]), "v"(e[4]), "v"(f[0]), "v"(f[1]),
+                     "v"(f[2]), "v"(f[3]), "v"(f[4]));
+#pragma unroll
+        for (int n = 0; n < 5; ++n) {
+          dx[n] = a[n] - b[n];
+          dy[n] = e[n] - f[n];
+        }
+      }
 #pragma unroll
       for (int n = 0; n < 5; ++n) {
-        const real t = (sfx[n][r - 2][xl] - sfx[n][r - 2][xl + 1]) * dt;
+        const real t = dx[n] * dt;
         u[n] = u[n] + over_h(t, p2x, P.idx, P.dx);
       }
 #pragma unroll
       for (int n = 0; n < 5; ++n) {
-        const real t = (sfy[n][r - 2][xl] - sfy[n][r - 1][xl]) * dt;
+        const real t = dy[n] * dt;
         u[n] = u[n] + over_h(t, p2y, P.idy, P.dy);
       }
       real zl[5], zh[5];
