@@ -101,10 +101,9 @@ __device__ __forceinline__ double sqrt_f64(double x) {
 // same bits (measured on MI355X over 2.7e8 random pairs per exponent range
 // and every pair of special values), and quotients with one denominator can
 // share its refined reciprocal.  The square root likewise drops its scaling
-// for positive finite arguments of at least 2^-767 (the zero and infinity
-// fix goes too).  The
-// caller guarantees the operand ranges (the stage kernel checks every cell
-// it loads, cell_in_fast_range).
+// and its zero and infinity fix for positive finite arguments of at least
+// 2^-767.  The caller guarantees the operand ranges (the stage kernel tests
+// every cell it loads, cell_in_fast_range).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double rcp_refined(double d) {
   double r = __builtin_amdgcn_rcp(d);
@@ -317,7 +316,8 @@ __device__ __forceinline__ void hll(const Prim<real> L, const Prim<real> R, real
     pout = (real)0.5 * (L.p + R.p);
     const real SLSR = SL * SR, dS = SR - SL, iS = A::rcp(dS);
 #pragma unroll
-    for (int n = 0; n < 5; ++n) F[n] = A::divr(FL[n] * SR - FR[n] * SL + (UR[n] - UL[n]) * SLSR, dS, iS);
+    for (int n = 0; n < 5; ++n)
+      F[n] = A::divr(FL[n] * SR - FR[n] * SL + (UR[n] - UL[n]) * SLSR, dS, iS);
   }
 }
 

@@ -336,8 +336,9 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
 
   // ---- warm-up: planes k0-2 .. k0+1, z slope at k0 and the z face k0-1/2
   // (the z face always in the plain form).  bb: the lane's cell is outside
-  // the FAST range in planes k-1, k, k+1, k+2 (bits 3..0); hb1: so is its
-  // halo cell in plane k+1 (with viscosity, converted one plane ahead).
+  // the FAST range in planes k-1, k, k+1 (bits 2..0; plane k+2 joins in the
+  // face phase); hb1, with viscosity: so is its halo cell in the plane held
+  // in fz (plane k at the top of each plane).
   bool bm1, b0, b1, hb1 = false;
   const Prim<real> Pm1 = load_plane(k0 - 1, bm1);
   Prim<real> P0 = load_plane(k0, b0);
@@ -514,15 +515,16 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
       if (interior) {
         real F[5], po;
         face_flux4<real, FV3D_IX, RECON, RS, FA>(lds_prim(r, c - 2), lds_prim(r, c - 1), P0,
-                                             lds_prim(r, c + 1), P, F, po);
+                                                 lds_prim(r, c + 1), P, F, po);
 #pragma unroll
         for (int n = 0; n < 5; ++n) sfx[n][r - 2][xl] = F[n];
       } else if (xedge) {
         // the right face of the last interior column for row lane+2
         real F[5], po;
-        face_flux4<real, FV3D_IX, RECON, RS, FA>(lds_prim(lane + 2, WX), lds_prim(lane + 2, WX + 1),
-                                             lds_prim(lane + 2, WX + 2),
-                                             lds_prim(lane + 2, WX + 3), P, F, po);
+        face_flux4<real, FV3D_IX, RECON, RS, FA>(lds_prim(lane + 2, WX),
+                                                 lds_prim(lane + 2, WX + 1),
+                                                 lds_prim(lane + 2, WX + 2),
+                                                 lds_prim(lane + 2, WX + 3), P, F, po);
 #pragma unroll
         for (int n = 0; n < 5; ++n) sfx[n][lane][WX] = F[n];
       }
@@ -536,8 +538,8 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
           const int dr = isy ? 1 : 0, dc = isy ? 0 : 1;
           real F[5], po;
           face_flux_xy<real, RECON, RS, FA>(lds_prim(r0, c0), lds_prim(r0 + dr, c0 + dc),
-                                        lds_prim(r0 + 2 * dr, c0 + 2 * dc),
-                                        lds_prim(r0 + 3 * dr, c0 + 3 * dc), isy, P, F, po);
+                                            lds_prim(r0 + 2 * dr, c0 + 2 * dc),
+                                            lds_prim(r0 + 3 * dr, c0 + 3 * dc), isy, P, F, po);
           if (isy) {
 #pragma unroll
             for (int n = 0; n < 5; ++n) sfy[n][r - 2][xl] = F[n];
@@ -551,7 +553,7 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
       if (yworker && !mwave) {
         real F[5], po;
         face_flux4<real, FV3D_IY, RECON, RS, FA>(lds_prim(r - 2, c), lds_prim(r - 1, c), P0,
-                                             lds_prim(r + 1, c), P, F, po);
+                                                 lds_prim(r + 1, c), P, F, po);
 #pragma unroll
         for (int n = 0; n < 5; ++n) sfy[n][r - 2][xl] = F[n];
       }
@@ -585,8 +587,8 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
             qR.p = P1.p - P1.r * (real)P.g * (real)P.dz * (real)0.5;
           }
         }
-        riemann<real, RS, FZ>(rotate<real, FV3D_IZ>(qL), rotate<real, FV3D_IZ>(qR), (real)P.gamma, gm1,
-                          (real)P.entho, Fzh, pouth);
+        riemann<real, RS, FZ>(rotate<real, FV3D_IZ>(qL), rotate<real, FV3D_IZ>(qR),
+                              (real)P.gamma, gm1, (real)P.entho, Fzh, pouth);
         unrotate_flux<real, FV3D_IZ>(Fzh);
       }
     };
