@@ -189,11 +189,12 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
   const bool pcm_wb = !LEAN && P.pcm_wb, gravity = P.gravity;
   const bool p2x = POW2 || P.pow2x, p2y = POW2 || P.pow2y, p2z = POW2 || P.pow2z;
   const double* kztab = LEAN ? nullptr : P.kztab;
-  // The primitive plane (and the temperature plane) are double-buffered by
-  // plane parity unless viscosity is on, so the next plane can be published
-  // while slower waves still read this one; viscosity keeps one buffer (its
-  // extra planes leave no LDS for a second) and a barrier instead.
-  constexpr int NBUF = VISC ? 1 : 2;
+  // The primitive plane, the temperature plane and the k-1, k+1 velocity
+  // planes are double-buffered by plane parity, so the next plane can be
+  // published while slower waves still read this one, with no barrier
+  // before the publish (the viscous forms, one workgroup per CU, take up to
+  // 143 KB of LDS with both copies).
+  constexpr int NBUF = 2;
   // viscosity: FP32 evaluates each x and y face once in the face phase
   // (VFACE); FP64 evaluates a cell's six sides in the update from one
   // 3x3x3 stencil, whose shared LDS reads cost less than re-reading each
@@ -209,8 +210,8 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
   constexpr bool FASTK = sizeof(real) == 8 && RECON == FV3D_PLM && RS == FV3D_HLL && !C91Z;
   __shared__ real sqb[NBUF][5][NR][NC];
   __shared__ real sTb[NBUF][TC ? NR : 1][TC ? NC : 1];
-  __shared__ real svm[VISC ? 3 : 1][VISC ? NR : 1][VISC ? NC : 1];
-  __shared__ real svp[VISC ? 3 : 1][VISC ? NR : 1][VISC ? NC : 1];
+  __shared__ real svmb[VISC ? NBUF : 1][VISC ? 3 : 1][VISC ? NR : 1][VISC ? NC : 1];
+  __shared__ real svpb[VISC ? NBUF : 1][VISC ? 3 : 1][VISC ? NR : 1][VISC ? NC : 1];
   __shared__ real sfx[5][TY][WX + 1];
   __shared__ real sfy[5][TY + 1][WX];
   // viscosity: mu * (stress terms) of each x face and y face, evaluated once
@@ -355,12 +356,14 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
   } else if (hown) {
     if (VISC) {
       // halo column ring: fz = primitives of plane k, cz = raw values of
-      // plane k+1, and the cell's LDS slot holds the velocities of plane k-1
+      // plane k+1, and the cell's slot in the other plane buffer holds the
+      // velocities of plane k-1
       const Prim<real> A = load_halo(k0, hb1);
       fz[0] = A.r; fz[1] = A.u; fz[2] = A.v; fz[3] = A.w; fz[4] = A.p;
       bool bbm;
       const Prim<real> B = load_halo(k0 - 1, bbm);
-      sqb[0][1][hr][hc] = B.u; sqb[0][2][hr][hc] = B.v; sqb[0][3][hr][hc] = B.w;
+      real (*sqm)[NR][NC] = sqb[(k0 - 1) & 1];
+      sqm[1][hr][hc] = B.u; sqm[2][hr][hc] = B.v; sqm[3][hr][hc] = B.w;
       fetch_halo_any(k0 + 1, cz);
     } else {
       fetch_halo(k0, cz);
@@ -379,8 +382,10 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
   int bad = 0;
 
   for (int k = k0; k < k1; ++k) {
-    real (*sq)[NR][NC] = sqb[NBUF == 2 ? (k & 1) : 0];
-    real (*sT)[TC ? NC : 1] = sTb[NBUF == 2 ? (k & 1) : 0];
+    real (*sq)[NR][NC] = sqb[k & 1];
+    real (*sT)[TC ? NC : 1] = sTb[k & 1];
+    real (*svm)[VISC ? NR : 1][VISC ? NC : 1] = svmb[VISC ? (k & 1) : 0];
+    real (*svp)[VISC ? NR : 1][VISC ? NC : 1] = svpb[VISC ? (k & 1) : 0];
     auto lds_prim = [&](int rr, int cc) -> Prim<real> {
       return Prim<real>{sq[0][rr][cc], sq[1][rr][cc], sq[2][rr][cc], sq[3][rr][cc], sq[4][rr][cc]};
     };
@@ -427,7 +432,8 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
     if (VISC && hown) {
       H = Prim<real>{fz[0], fz[1], fz[2], fz[3], fz[4]};
       hbad = hb1;
-      Hm.u = sq[1][hr][hc]; Hm.v = sq[2][hr][hc]; Hm.w = sq[3][hr][hc];  // plane k-1
+      real (*sqm)[NR][NC] = sqb[(k - 1) & 1];  // plane k-1
+      Hm.u = sqm[1][hr][hc]; Hm.v = sqm[2][hr][hc]; Hm.w = sqm[3][hr][hc];
       Hp = halo_prim(k + 1, cz, hb1);
     } else if (hown) {
       bool hfz;
@@ -438,7 +444,6 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
       if (hfz) H.w = -H.w;
     }
 
-    if (NBUF == 1) __syncthreads();  // readers of the previous plane are done
     sq[0][r][c] = P0.r; sq[1][r][c] = P0.u; sq[2][r][c] = P0.v; sq[3][r][c] = P0.w;
     sq[4][r][c] = P0.p;
     if (TC) sT[r][c] = T0;
