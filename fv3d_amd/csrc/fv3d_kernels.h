@@ -8,10 +8,13 @@ namespace fv3d_hip {
 
 // Stage tile shapes: WX columns (64: one window row per wave, or 32: two
 // rows per wave) by stage_ty rows.  Without viscosity: 64 x 8 or 32 x 16
-// (12 or 10 waves, 168 VGPRs).  With viscosity, FP32 keeps those shapes
-// (no spills); FP64 takes 64 x 4 or 32 x 12 (8 waves, 255 of 256 VGPRs, no
-// spills), where 64 x 8 spills about 80 VGPRs and runs 27% slower on the
-// stratified box.
+// (12 or 10 waves, a 168-VGPR budget).  With viscosity, FP32 keeps those
+// shapes; FP64 takes 64 x 4 or 32 x 12 (8 waves, a 256-VGPR budget), where
+// 64 x 8 spills about 80 VGPRs and runs 27% slower on the stratified box.
+// Spilled VGPRs of the lean single-box kernels: FP32 none; FP64 none at
+// 64 x 8 without conduction or viscosity (9 for HLL at 32 x 16), 8-30 with
+// conduction alone, and with viscosity none for HLLC and 4-6 for HLL with
+// conduction.
 template <typename real, bool VISC, int WX>
 constexpr int stage_ty_of() {
   return WX == 64 ? ((VISC && sizeof(real) == 8) ? 4 : 8) : ((VISC && sizeof(real) == 8) ? 12 : 16);
