@@ -571,21 +571,42 @@ int fv3d_batch_tune_shape(fv3d_batch* b, int nsteps, const char* cache_path, int
     b->step = step0;
     return FV3D_OK;
   };
-  int best_nzc = 0, best_grid = b->num_cus, best_wx = widths[0];
-  float best = 1e30f;
+  // every candidate once, then the four fastest again, twice each over twice
+  // the steps: one short timing per candidate picked a slower shape in about
+  // one run of four (the 512-box RT ensemble: 21.8 against 22.05 Gcell/s)
+  struct Cand {
+    float ms;
+    int nzc, grid, wx;
+  };
+  std::vector<Cand> cands;
+  auto time_cand = [&](const Cand& c, int steps, float& ms) -> int {
+    b->wx = c.wx;
+    if (int rc = fv3d_batch_configure(b, c.nzc, c.grid)) return rc;
+    if (int rc = fv3d_batch_time(b, steps, FV3D_BATCH_GRAPH, &ms)) return rc;
+    return restore();
+  };
   for (int wx : widths) {
-    b->wx = wx;
     for (int nzc : {8, 16, 32, 64, 128}) {
       for (int mult : {1, 2, 4}) {
-        const int grid = b->num_cus * mult;
-        if (int rc = fv3d_batch_configure(b, nzc, grid)) return rc;
-        float ms = 0.f;
-        if (int rc = fv3d_batch_time(b, nsteps, FV3D_BATCH_GRAPH, &ms)) return rc;
-        if (int rc = restore()) return rc;
-        if (ms < best) { best = ms; best_nzc = nzc; best_grid = grid; best_wx = wx; }
+        Cand c{0.f, nzc, b->num_cus * mult, wx};
+        if (int rc = time_cand(c, nsteps, c.ms)) return rc;
+        cands.push_back(c);
       }
     }
   }
+  std::sort(cands.begin(), cands.end(), [](const Cand& x, const Cand& y) { return x.ms < y.ms; });
+  Cand best{1e30f, cands[0].nzc, cands[0].grid, cands[0].wx};
+  for (size_t i = 0; i < cands.size() && i < 4; ++i) {
+    Cand c = cands[i];
+    c.ms = 1e30f;
+    for (int rep = 0; rep < 2; ++rep) {
+      float ms = 0.f;
+      if (int rc = time_cand(c, 2 * nsteps, ms)) return rc;
+      c.ms = std::min(c.ms, ms);
+    }
+    if (c.ms < best.ms) best = c;
+  }
+  const int best_nzc = best.nzc, best_grid = best.grid, best_wx = best.wx;
   if (cache_path && *cache_path) {
     std::ofstream outf(cache_path, std::ios::app);
     outf << key << " " << best_nzc << " " << best_grid << " " << best_wx << "\n";

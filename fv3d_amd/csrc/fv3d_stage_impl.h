@@ -200,21 +200,22 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
   // 3x3x3 stencil, whose shared LDS reads cost less than re-reading each
   // face's stencil (measured 8% faster on the stratified box)
   constexpr bool VFACE = VISC && sizeof(real) == 4;
-  // 64-column tiles without conduction or viscosity (FINE): the face phase
-  // and the update are ordered per row instead of by a barrier.  A row's
-  // update reads its own faces, the y faces above it and the right edge's x
-  // faces; the wave that writes the y faces above a row, and the wave that
-  // writes the edge, each raise a flag (the plane index) in LDS after them,
-  // and the row's wave waits for those two flags.  Older waves then start
-  // their update while younger ones still solve faces.  The two edge jobs
-  // (the top row of y faces, the right x face of every interior row) go to
-  // the two oldest waves, halo rows 0 and 1, and every wave solves its y
-  // faces first, so the flags rise early.  Measured on the 256^3 blast:
-  // FP64 10.78 -> 10.97 Gcell/s, FP32 compensated 23.5 -> 24.3; the FP32
-  // plain stages (26.1 -> 25.6) and the viscous kernels (stratified box
-  // FP64 6.09 -> 5.87) ran slower with it and keep the barrier.
-  constexpr bool FINE =
-      RPW == 1 && !TC && !VISC && (sizeof(real) == 8 || MODE != STAGE_PLAIN);
+  // Without conduction or viscosity (FINE) the face phase and the update
+  // are ordered per row instead of by a barrier.  A row's update reads its
+  // own faces, the y faces above it and the right edge's x faces; the wave
+  // that writes the y faces above a row, and the wave that writes the edge,
+  // each raise a flag (the plane index) in LDS after them, and the row's
+  // wave waits for those two flags.  Older waves then start their update
+  // while younger ones still solve faces.  The edge jobs (the top row of y
+  // faces, the right x face of every interior row) go to the oldest waves
+  // (halo rows 0 and 1: waves 0 and 1 of 64-column tiles, wave 0 of 32-column
+  // ones), and every wave solves its y faces first, so the flags rise early.
+  // Measured on the 256^3 blast: FP64 10.78 -> 10.97 Gcell/s, FP32
+  // compensated 23.5 -> 24.3; on the 512-box RT ensemble (32-column tiles,
+  // FP32 compensated) 21.67 -> 22.05; the FP32 plain stages (26.1 -> 25.6)
+  // and the viscous kernels (stratified box FP64 6.09 -> 5.87) ran slower
+  // with it and keep the barrier.
+  constexpr bool FINE = !TC && !VISC && (sizeof(real) == 8 || MODE != STAGE_PLAIN);
   // FP64 PLM-HLL faces (not C91, whose ghost planes are not range-checked)
   // take the FAST quotients and square roots in every plane where all the
   // cells they read are in the FAST range (cell_in_fast_range); other
@@ -265,7 +266,7 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
   // wave-uniform: interior rows 2..TY+1 start and end on wave boundaries
   const bool interior = r >= 2 && r < TY + 2;
   const bool yworker = r >= 2 && r < TY + (FINE ? 2 : 3);
-  const bool ytop = FINE && wave == 1;  // the top row of y faces (FINE)
+  const bool ytop = FINE && RPW == 1 && wave == 1;  // the top row of y faces
   const bool store = interior && gi < P.Nx && gj < P.Ny;
   const bool hwave = RPW == 1 ? (wave < 2 || wave >= NW - 2) : (wave == 0 || wave == NW - 1);
   const int hidx = RPW == 1 ? (wave < 2 ? wave : wave - (NW - 4))
@@ -276,14 +277,14 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
   const int hc = hidx < 2 ? hidx : WX + hidx;
   // the right x face of the last interior column, row lane+2.  64-column
   // tiles: wave 0 (FINE) or the last wave, which computes no other face.
-  // 32-column tiles (MERGE): the last wave, lanes 32..32+TY-1, in the same
-  // face evaluation as the top row of y faces in its lanes 0-31; with
-  // viscosity in FP64 (where the merged evaluation spills 7 more VGPRs) wave
-  // 0, lanes < TY
+  // 32-column tiles (MERGE): wave 0 (FINE) or the last wave, lanes
+  // 32..32+TY-1, in the same face evaluation as the top row of y faces in
+  // its lanes 0-31; with viscosity in FP64 (where the merged evaluation
+  // spills 7 more VGPRs) wave 0, lanes < TY
   constexpr bool MERGE = RPW == 2 && !(VISC && sizeof(real) == 8);
   static_assert(!MERGE || TY <= 32, "the edge faces take lanes 32..32+TY-1");
   const bool xedge = (RPW == 1 && !FINE ? wave == NW - 1 : !MERGE && wave == 0) && lane < TY;
-  const bool mwave = MERGE && wave == NW - 1;
+  const bool mwave = MERGE && wave == (FINE ? 0 : NW - 1);
   bool hfx, hfy;
   const int hsi = map_coord(i0 - 2 + hc, P.Nx, P.bcx, hfx);
   const int hsj = map_coord(j0 - 2 + hr, P.Ny, P.bcy, hfy);
@@ -538,11 +539,11 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
     Prim<real> sK1{};
     // The x and y face solves, in the FAST form (FASTK, every cell of plane k
     // in range: uniform over the workgroup) or the plain one.
-    // FINE: raise flag f to this plane once the wave's LDS writes before it
-    // are complete
-    auto raise = [&](int* f) {
+    // FINE: raise flag f to this plane (stored by the lanes lead) once the
+    // wave's LDS writes before it are complete
+    auto raise = [&](int* f, bool lead) {
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-      if (lane == 0) __hip_atomic_store(f, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (lead) __hip_atomic_store(f, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     };
     auto face_xy = [&](auto fast_tag) {
       constexpr bool FA = decltype(fast_tag)::value;
@@ -565,8 +566,9 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
       };
       if constexpr (FINE) {
         y_faces();
-        if (yworker) raise(&syf[r - 2]);
-        else if (ytop) raise(&syf[TY]);
+        // one flag per row: lane 0 of each row (lanes 0 and 32 of 32-column tiles)
+        if (yworker) raise(&syf[r - 2], (lane & (WX - 1)) == 0);
+        else if (ytop) raise(&syf[TY], lane == 0);
       }
       // ---- x faces: face between window columns c-1 and c for every interior row
       if (interior) {
@@ -591,7 +593,7 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
         const bool isy = lane < 32;
         const int er = lane - 32 + 2;
         if (isy || er < TY + 2) {
-          const int r0 = isy ? r - 2 : er, c0 = isy ? c : WX;
+          const int r0 = isy ? TY : er, c0 = isy ? c : WX;
           const int dr = isy ? 1 : 0, dc = isy ? 0 : 1;
           real F[5], po;
           face_flux_xy<real, RECON, RS, FA>(lds_prim(r0, c0), lds_prim(r0 + dr, c0 + dc),
@@ -599,7 +601,7 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
                                             lds_prim(r0 + 3 * dr, c0 + 3 * dc), isy, P, F, po);
           if (isy) {
 #pragma unroll
-            for (int n = 0; n < 5; ++n) sfy[n][r - 2][xl] = F[n];
+            for (int n = 0; n < 5; ++n) sfy[n][TY][xl] = F[n];
           } else {
 #pragma unroll
             for (int n = 0; n < 5; ++n) sfx[n][er - 2][WX] = F[n];
@@ -607,7 +609,13 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
         }
       }
       if constexpr (FINE) {
-        if (wave == 0) raise(&sxf);
+        if (wave == 0) {  // the edge x faces (and, merged, the top row of y faces)
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+          if (lane == 0) {
+            __hip_atomic_store(&sxf, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (MERGE) __hip_atomic_store(&syf[TY], k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          }
+        }
       } else {
         y_faces();
       }
