@@ -210,12 +210,13 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
   // faces, the right x face of every interior row) go to the oldest waves
   // (halo rows 0 and 1: waves 0 and 1 of 64-column tiles, wave 0 of 32-column
   // ones), and every wave solves its y faces first, so the flags rise early.
-  // Measured on the 256^3 blast: FP64 10.78 -> 10.97 Gcell/s, FP32
-  // compensated 23.5 -> 24.3; on the 512-box RT ensemble (32-column tiles,
-  // FP32 compensated) 21.67 -> 22.05; the FP32 plain stages (26.1 -> 25.6)
-  // and the viscous kernels (stratified box FP64 6.09 -> 5.87) ran slower
-  // with it and keep the barrier.
-  constexpr bool FINE = !TC && !VISC && (sizeof(real) == 8 || MODE != STAGE_PLAIN);
+  // Measured on the 256^3 blast: FP64 10.78 -> 10.97 Gcell/s; the FP32
+  // compensated second stage 403 -> 374 us (the 512-box RT ensemble, 32-column
+  // tiles: 2970 -> 2758 us).  The first compensated stage (306 -> 310 us,
+  // ensemble 2151 -> 2337), the FP32 plain stages (26.1 -> 25.6 Gcell/s) and
+  // the viscous kernels (stratified box FP64 6.09 -> 5.87) ran slower with
+  // it and keep the barrier.
+  constexpr bool FINE = !TC && !VISC && (sizeof(real) == 8 || MODE == STAGE_INC2);
   // FP64 PLM-HLL faces (not C91, whose ghost planes are not range-checked)
   // take the FAST quotients and square roots in every plane where all the
   // cells they read are in the FAST range (cell_in_fast_range); other
