@@ -200,23 +200,28 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
   // 3x3x3 stencil, whose shared LDS reads cost less than re-reading each
   // face's stencil (measured 8% faster on the stratified box)
   constexpr bool VFACE = VISC && sizeof(real) == 4;
-  // Without conduction or viscosity (FINE) the face phase and the update
-  // are ordered per row instead of by a barrier.  A row's update reads its
-  // own faces, the y faces above it and the right edge's x faces; the wave
-  // that writes the y faces above a row, and the wave that writes the edge,
-  // each raise a flag (the plane index) in LDS after them, and the row's
-  // wave waits for those two flags.  Older waves then start their update
-  // while younger ones still solve faces.  The edge jobs (the top row of y
-  // faces, the right x face of every interior row) go to the oldest waves
-  // (halo rows 0 and 1: waves 0 and 1 of 64-column tiles, wave 0 of 32-column
-  // ones), and every wave solves its y faces first, so the flags rise early.
-  // Measured on the 256^3 blast: FP64 10.78 -> 10.97 Gcell/s; the FP32
-  // compensated second stage 403 -> 374 us (the 512-box RT ensemble, 32-column
-  // tiles: 2970 -> 2758 us).  The first compensated stage (306 -> 310 us,
-  // ensemble 2151 -> 2337), the FP32 plain stages (26.1 -> 25.6 Gcell/s) and
-  // the viscous kernels (stratified box FP64 6.09 -> 5.87) ran slower with
-  // it and keep the barrier.
-  constexpr bool FINE = !TC && !VISC && (sizeof(real) == 8 || MODE == STAGE_INC2);
+  // FINE: the face phase and the update are ordered per row instead of by a
+  // barrier.  A row's update reads its own faces, the y faces above it and
+  // the right edge's x faces (with viscosity and conduction, also the planes
+  // published before the face phase); the wave that writes the y faces above
+  // a row, and the wave that writes the edge, each raise a flag (the plane
+  // index) in LDS after them, and the row's wave waits for those two flags.
+  // Older waves then start their update while younger ones still solve
+  // faces, and every wave solves its y faces first, so the flags rise early.
+  // Taken by the FP64 kernels (64-column tiles with viscosity) and the
+  // second compensated FP32 stage.  Measured: 256^3 blast FP64 10.8 -> 11.2
+  // Gcell/s, FP32 compensated second stage 403 -> 374 us (512-box RT
+  // ensemble, 32-column tiles: 2970 -> 2758 us), stratified box FP64 6.10 ->
+  // 6.39, C5 forward run 68.0 -> 63.7 ms.  The first compensated stage (306 ->
+  // 310 us, ensemble 2151 -> 2337), the FP32 plain stages (26.1 -> 25.6
+  // Gcell/s) and the FP32 viscous forms keep the barrier.  MOVE: on
+  // 32-column tiles the merged edge job goes from the last wave to wave 0,
+  // the oldest, whose flags then rise first (ensemble 22.47 -> 22.57
+  // Gcell/s); on 64-column tiles moving the edge jobs to the oldest waves ran
+  // slower (FP64 blast -4%, stratified box -8%).
+  constexpr bool FINE = (!TC && !VISC && (sizeof(real) == 8 || MODE == STAGE_INC2)) ||
+                        (VISC && sizeof(real) == 8 && RPW == 1);
+  constexpr bool MOVE = FINE && RPW == 2;
   // FP64 PLM-HLL faces (not C91, whose ghost planes are not range-checked)
   // take the FAST quotients and square roots in every plane where all the
   // cells they read are in the FAST range (cell_in_fast_range); other
@@ -266,8 +271,9 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
 
   // wave-uniform: interior rows 2..TY+1 start and end on wave boundaries
   const bool interior = r >= 2 && r < TY + 2;
-  const bool yworker = r >= 2 && r < TY + (FINE ? 2 : 3);
-  const bool ytop = FINE && RPW == 1 && wave == 1;  // the top row of y faces
+  // rows whose y face below them the row's own wave solves (the top row of
+  // y faces too, unless the merged edge job holds it)
+  const bool yworker = r >= 2 && r < TY + (MOVE ? 2 : 3);
   const bool store = interior && gi < P.Nx && gj < P.Ny;
   const bool hwave = RPW == 1 ? (wave < 2 || wave >= NW - 2) : (wave == 0 || wave == NW - 1);
   const int hidx = RPW == 1 ? (wave < 2 ? wave : wave - (NW - 4))
@@ -277,15 +283,16 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
   const int hr = hl < NR ? hl : 0;
   const int hc = hidx < 2 ? hidx : WX + hidx;
   // the right x face of the last interior column, row lane+2.  64-column
-  // tiles: wave 0 (FINE) or the last wave, which computes no other face.
-  // 32-column tiles (MERGE): wave 0 (FINE) or the last wave, lanes
-  // 32..32+TY-1, in the same face evaluation as the top row of y faces in
-  // its lanes 0-31; with viscosity in FP64 (where the merged evaluation
-  // spills 7 more VGPRs) wave 0, lanes < TY
+  // tiles: the last wave, which computes no other face.  32-column tiles
+  // (MERGE): the last wave, or wave 0 (MOVE), lanes 32..32+TY-1, in the same
+  // face evaluation as the top row of y faces in its lanes 0-31; with
+  // viscosity in FP64 (where the merged evaluation spills 7 more VGPRs) wave
+  // 0, lanes < TY
   constexpr bool MERGE = RPW == 2 && !(VISC && sizeof(real) == 8);
   static_assert(!MERGE || TY <= 32, "the edge faces take lanes 32..32+TY-1");
-  const bool xedge = (RPW == 1 && !FINE ? wave == NW - 1 : !MERGE && wave == 0) && lane < TY;
-  const bool mwave = MERGE && wave == (FINE ? 0 : NW - 1);
+  static_assert(!MOVE || MERGE, "the moved edge job is the merged one");
+  const bool xedge = (RPW == 1 ? wave == NW - 1 : !MERGE && wave == 0) && lane < TY;
+  const bool mwave = MERGE && wave == (MOVE ? 0 : NW - 1);
   bool hfx, hfy;
   const int hsi = map_coord(i0 - 2 + hc, P.Nx, P.bcx, hfx);
   const int hsj = map_coord(j0 - 2 + hr, P.Ny, P.bcy, hfy);
@@ -556,20 +563,13 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
                                                    lds_prim(r + 1, c), P, F, po);
 #pragma unroll
           for (int n = 0; n < 5; ++n) sfy[n][r - 2][xl] = F[n];
-        } else if (ytop) {  // between rows TY+1 and TY+2
-          real F[5], po;
-          face_flux4<real, FV3D_IY, RECON, RS, FA>(lds_prim(TY, c), lds_prim(TY + 1, c),
-                                                   lds_prim(TY + 2, c), lds_prim(TY + 3, c), P, F,
-                                                   po);
-#pragma unroll
-          for (int n = 0; n < 5; ++n) sfy[n][TY][xl] = F[n];
         }
       };
       if constexpr (FINE) {
         y_faces();
-        // one flag per row: lane 0 of each row (lanes 0 and 32 of 32-column tiles)
-        if (yworker) raise(&syf[r - 2], (lane & (WX - 1)) == 0);
-        else if (ytop) raise(&syf[TY], lane == 0);
+        // one flag per row: lane 0 of each row (lanes 0 and 32 of 32-column
+        // tiles); the merged top row rises with the edge below
+        if (yworker && !mwave) raise(&syf[r - 2], (lane & (WX - 1)) == 0);
       }
       // ---- x faces: face between window columns c-1 and c for every interior row
       if (interior) {
@@ -610,7 +610,7 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
         }
       }
       if constexpr (FINE) {
-        if (wave == 0) {  // the edge x faces (and, merged, the top row of y faces)
+        if (RPW == 1 ? wave == NW - 1 : mwave) {  // the edge x faces (and, merged, the top row of y faces)
           __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
           if (lane == 0) {
             __hip_atomic_store(&sxf, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
