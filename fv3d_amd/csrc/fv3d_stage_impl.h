@@ -208,19 +208,20 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
   // index) in LDS after them, and the row's wave waits for those two flags.
   // Older waves then start their update while younger ones still solve
   // faces, and every wave solves its y faces first, so the flags rise early.
-  // Taken by the FP64 kernels (64-column tiles with viscosity) and the
-  // second compensated FP32 stage.  Measured: 256^3 blast FP64 10.8 -> 11.2
-  // Gcell/s, FP32 compensated second stage 403 -> 374 us (512-box RT
-  // ensemble, 32-column tiles: 2970 -> 2758 us), stratified box FP64 6.10 ->
-  // 6.39, C5 forward run 68.0 -> 63.7 ms.  The first compensated stage (306 ->
-  // 310 us, ensemble 2151 -> 2337), the FP32 plain stages (26.1 -> 25.6
-  // Gcell/s) and the FP32 viscous forms keep the barrier.  MOVE: on
+  // Taken by the FP64 kernels, the second compensated FP32 stage and the
+  // viscous kernels of 64-column tiles (FP32: each row's viscous y face is
+  // solved with its y face, before the flag).  Measured: 256^3 blast FP64
+  // 10.8 -> 11.3 Gcell/s, FP32 compensated second stage 403 -> 369 us
+  // (512-box RT ensemble, 32-column tiles: 2970 -> 2746 us), stratified box
+  // FP64 6.10 -> 6.39 and FP32 15.36 -> 15.93, C5 forward run 68.0 -> 63.7
+  // ms.  The first compensated stage (306 -> 310 us, ensemble 2151 -> 2337)
+  // and the FP32 plain stages (26.1 -> 25.6 Gcell/s) keep the barrier.  MOVE: on
   // 32-column tiles the merged edge job goes from the last wave to wave 0,
   // the oldest, whose flags then rise first (ensemble 22.47 -> 22.57
   // Gcell/s); on 64-column tiles moving the edge jobs to the oldest waves ran
   // slower (FP64 blast -4%, stratified box -8%).
   constexpr bool FINE = (!TC && !VISC && (sizeof(real) == 8 || MODE == STAGE_INC2)) ||
-                        (VISC && sizeof(real) == 8 && RPW == 1);
+                        (VISC && RPW == 1);
   constexpr bool MOVE = FINE && RPW == 2;
   // FP64 PLM-HLL faces (not C91, whose ghost planes are not range-checked)
   // take the FAST quotients and square roots in every plane where all the
@@ -567,6 +568,12 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
       };
       if constexpr (FINE) {
         y_faces();
+        if (VFACE && yworker) {  // the viscous y face below the row, before its flag
+          real t[5];
+          visc_y_side<real>(vstencil(r, c), 1, oodx, oody, oodz, t);
+#pragma unroll
+          for (int n = 1; n < 5; ++n) svy[n - 1][r - 2][xl] = mu * t[n];
+        }
         // one flag per row: lane 0 of each row (lanes 0 and 32 of 32-column
         // tiles); the merged top row rises with the edge below
         if (yworker && !mwave) raise(&syf[r - 2], (lane & (WX - 1)) == 0);
@@ -587,6 +594,12 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
                                                  lds_prim(lane + 2, WX + 3), P, F, po);
 #pragma unroll
         for (int n = 0; n < 5; ++n) sfx[n][lane][WX] = F[n];
+        if (FINE && VFACE) {  // its viscous face, before the edge flag
+          real t[5];
+          visc_x_side<real>(vstencil(lane + 2, WX + 2), 1, oodx, oody, oodz, t);
+#pragma unroll
+          for (int n = 1; n < 5; ++n) svx[n - 1][lane][WX] = mu * t[n];
+        }
       }
       if (mwave) {
         // top y faces (lanes 0-31: row TY+2) and edge x faces (lanes 32..:
@@ -669,7 +682,7 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
         visc_x_side<real>(vstencil(r, c), 1, oodx, oody, oodz, t);
 #pragma unroll
         for (int n = 1; n < 5; ++n) svx[n - 1][r - 2][xl] = mu * t[n];
-      } else if (xedge) {
+      } else if (xedge && !FINE) {
         real t[5];
         visc_x_side<real>(vstencil(lane + 2, WX + 2), 1, oodx, oody, oodz, t);
 #pragma unroll
@@ -680,7 +693,7 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
 #pragma unroll
         for (int n = 1; n < 5; ++n) svx[n - 1][lane - 32][WX] = mu * t[n];
       }
-      if (yworker) {
+      if (yworker && !FINE) {
         real t[5];
         visc_y_side<real>(vstencil(r, c), 1, oodx, oody, oodz, t);
 #pragma unroll
