@@ -208,21 +208,20 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
   // index) in LDS after them, and the row's wave waits for those two flags.
   // Older waves then start their update while younger ones still solve
   // faces, and every wave solves its y faces first, so the flags rise early.
-  // Taken by the FP64 kernels, the second compensated FP32 stage and the
-  // viscous kernels of 64-column tiles (FP32: each row's viscous y face is
-  // solved with its y face, before the flag).  Measured: 256^3 blast FP64
-  // 10.8 -> 11.3 Gcell/s, FP32 compensated second stage 403 -> 369 us
-  // (512-box RT ensemble, 32-column tiles: 2970 -> 2746 us), stratified box
-  // FP64 6.10 -> 6.39 and FP32 15.36 -> 15.93, C5 forward run 68.0 -> 63.7
-  // ms.  The first compensated stage (306 -> 310 us, ensemble 2151 -> 2337)
-  // and the FP32 plain stages (26.1 -> 25.6 Gcell/s) keep the barrier.  MOVE: on
-  // 32-column tiles the merged edge job goes from the last wave to wave 0,
-  // the oldest, whose flags then rise first (ensemble 22.47 -> 22.57
-  // Gcell/s); on 64-column tiles moving the edge jobs to the oldest waves ran
-  // slower (FP64 blast -4%, stratified box -8%).
-  constexpr bool FINE = (!TC && !VISC && (sizeof(real) == 8 || MODE == STAGE_INC2)) ||
-                        (VISC && RPW == 1);
-  constexpr bool MOVE = FINE && RPW == 2;
+  // Taken by every kernel without conduction or viscosity and by the viscous
+  // kernels of 64-column tiles (FP32: each row's viscous y face is solved
+  // with its y face, before the flag); conduction alone keeps the barrier
+  // (not measured).  Measured: 256^3 blast FP64 10.8 -> 11.3 Gcell/s, FP32
+  // compensated 23.5 -> 25.3, FP32 plain 26.1 -> 27.6; stratified box FP64
+  // 6.10 -> 6.39 and FP32 15.36 -> 15.93; C5 forward run 68.0 -> 63.7 ms;
+  // 512-box RT ensemble (32-column tiles) 21.6 -> 22.7.  MOVE: on 32-column
+  // tiles the merged edge job of the FP64 and second compensated stages goes
+  // from the last wave to wave 0, the oldest, whose flags then rise first
+  // (ensemble 22.47 -> 22.57; the first compensated stage ran slower with
+  // it, 2151 -> 2337 us).  On 64-column tiles moving the edge jobs to the
+  // oldest waves ran slower (FP64 blast -4%, stratified box -8%).
+  constexpr bool FINE = (!TC && !VISC) || (VISC && RPW == 1);
+  constexpr bool MOVE = FINE && RPW == 2 && (sizeof(real) == 8 || MODE == STAGE_INC2);
   // FP64 PLM-HLL faces (not C91, whose ghost planes are not range-checked)
   // take the FAST quotients and square roots in every plane where all the
   // cells they read are in the FAST range (cell_in_fast_range); other
