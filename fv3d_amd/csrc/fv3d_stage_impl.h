@@ -622,11 +622,13 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
         }
       }
       if constexpr (FINE) {
-        if (RPW == 1 ? wave == NW - 1 : mwave) {  // the edge x faces (and, merged, the top row of y faces)
+        // the edge x faces (and, merged, the top row of y faces)
+        if (RPW == 1 ? wave == NW - 1 : mwave) {
           __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
           if (lane == 0) {
             __hip_atomic_store(&sxf, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (MERGE) __hip_atomic_store(&syf[TY], k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (MERGE)
+              __hip_atomic_store(&syf[TY], k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
           }
         }
       } else {
