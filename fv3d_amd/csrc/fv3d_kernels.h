@@ -11,10 +11,10 @@ namespace fv3d_hip {
 // (12 or 10 waves, a 168-VGPR budget).  With viscosity, FP32 keeps those
 // shapes; FP64 takes 64 x 4 or 32 x 12 (8 waves, a 256-VGPR budget), where
 // 64 x 8 spills about 80 VGPRs and runs 27% slower on the stratified box.
-// Spilled VGPRs of the lean single-box kernels: FP32 none; FP64 none at
-// 64 x 8 without conduction or viscosity (9 for HLL at 32 x 16), 8-30 with
-// conduction alone, and with viscosity none for HLLC and 4-6 for HLL with
-// conduction.
+// Spilled VGPRs of the lean single-box kernels: FP32 none.  FP64 without
+// conduction or viscosity: none for HLLC and 2 for HLL at 64 x 8 (none and
+// 11-13 at 32 x 16); 8-31 with conduction alone; none with viscosity alone;
+// 4 (HLLC) or 6 (HLL) with viscosity and conduction at 64 x 4, 4 at 32 x 12.
 template <typename real, bool VISC, int WX>
 constexpr int stage_ty_of() {
   return WX == 64 ? ((VISC && sizeof(real) == 8) ? 4 : 8) : ((VISC && sizeof(real) == 8) ? 12 : 16);
