@@ -643,12 +643,26 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
         y_faces();
       }
     };
+    // The two extra face jobs of a 64-column tile, the top row of y faces
+    // (window row TY+2) and the right edge's x faces (the last wave), sit on
+    // the youngest waves, and a SIMD issues its oldest wave first: their
+    // flags rose late, and the top interior row and every update waited
+    // for them.  Those two waves solve their faces at priority 1, the others
+    // stay at 0 (the same bits).  FP64 blast 11.3 -> 11.8-11.9 Gcell/s, FP32
+    // plain 27.6 -> 28.3-28.7, stratified FP64 6.38 -> 6.57-6.63; the
+    // compensated FP32 forms level, and the FP32 viscous form 1% slower, so
+    // it is left out, as are the 32-column tiles (the ensemble 0.5% slower).
+    // Priority for every wave's y faces as well was slower (11.6 in FP64).
+    const bool edge_prio = FINE && RPW == 1 && (!VISC || sizeof(real) == 8) &&
+                           (wave == NW - 1 || r == TY + 2);
+    if (edge_prio) __builtin_amdgcn_s_setprio(1);
     if constexpr (FASTK) {
       if (fast) face_xy(std::true_type{});
       else face_xy(std::false_type{});
     } else {
       face_xy(std::false_type{});
     }
+    if (edge_prio) __builtin_amdgcn_s_setprio(0);
     // plane k+2, then the z face k+1/2 in the FAST form when the wave's
     // cells of planes k-1 .. k+2 are all in range (uniform over the wave)
     bool bad2;
