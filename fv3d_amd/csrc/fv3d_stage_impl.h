@@ -650,11 +650,15 @@ __device__ __forceinline__ void stage_tile(const real* __restrict__ Uin, const r
     // for them.  Those two waves solve their faces at priority 1, the others
     // stay at 0 (the same bits).  FP64 blast 11.3 -> 11.8-11.9 Gcell/s, FP32
     // plain 27.6 -> 28.3-28.7, stratified FP64 6.38 -> 6.57-6.63; the
-    // compensated FP32 forms level, and the FP32 viscous form 1% slower, so
-    // it is left out, as are the 32-column tiles (the ensemble 0.5% slower).
-    // Priority for every wave's y faces as well was slower (11.6 in FP64).
-    const bool edge_prio = FINE && RPW == 1 && (!VISC || sizeof(real) == 8) &&
-                           (wave == NW - 1 || r == TY + 2);
+    // compensated FP32 forms level.  The FP32 viscous form raises the top
+    // y-face wave only (both: 1% slower; the top one alone: stratified 16.46
+    // -> 16.67-16.76, 128^3 forward run 26.4-26.8 -> 25.4-25.5 ms).  The
+    // 32-column tiles keep priority 0 (the ensemble 0.5% slower with it).
+    // Priority for every wave's y faces as well was slower (11.6 in FP64),
+    // and for the top interior row's update level.
+    const bool edge_prio = FINE && RPW == 1 &&
+                           ((!VISC || sizeof(real) == 8) ? (wave == NW - 1 || r == TY + 2)
+                                                         : r == TY + 2);
     if (edge_prio) __builtin_amdgcn_s_setprio(1);
     if constexpr (FASTK) {
       if (fast) face_xy(std::true_type{});
